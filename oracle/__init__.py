@@ -1,0 +1,165 @@
+"""ctypes wrapper of the CPU oracle (libijkl_oracle.so, built from ijkl_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or as the timed host-core baseline.
+The product (cipsi_amd) never imports it.
+
+Each function restates the reference routine named in ijkl_oracle.c
+(CIPSI_sources/ijkl/transd.f, tpqkld.f, tijkld.f, tmono.f) for the C1 case.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libijkl_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        L.or_transd_c1.restype = ctypes.c_int64
+        L.or_transd_c1.argtypes = [ctypes.c_int, _dp, _ip, ctypes.c_int64, _dp]
+        for fn in (L.or_tpqkld_c1, L.or_tijkld_c1):
+            fn.restype = None
+            fn.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
+        for fn in (L.or_tpqkld_c1_range, L.or_tijkld_c1_range):
+            fn.restype = None
+            fn.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp,
+                           ctypes.c_int64, ctypes.c_int64]
+        for fn in (L.or_tpqkld_c1_omp_list, L.or_tijkld_c1_omp_list):
+            fn.restype = None
+            fn.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _lp,
+                           ctypes.c_int64, ctypes.c_int]
+        L.or_ijkl_c1.restype = ctypes.c_int
+        L.or_ijkl_c1.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
+        L.or_tmono.restype = None
+        L.or_tmono.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
+        L.or_energy.restype = ctypes.c_double
+        L.or_energy.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.or_energy_grad.restype = None
+        L.or_energy_grad.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
+                                     _dp, _dp, _dp, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _cmat(c):
+    c = np.asfortranarray(c, dtype=np.float64)
+    return c, c.shape[0]
+
+
+def n_pairs(n):
+    return n * (n + 1) // 2
+
+
+def transd(n_ao, xx, ix):
+    """C1 unpack of a label stream into the packed AO block (transd.f:85-97,210-218,356-373)."""
+    pa = n_pairs(n_ao)
+    out = np.empty(n_pairs(pa))
+    xx = _f64(xx)
+    ix = np.ascontiguousarray(ix, dtype=np.int32)
+    r = lib().or_transd_c1(n_ao, _d(xx), ix.ctypes.data_as(_ip), xx.size, _d(out))
+    if r < 0:
+        raise ValueError(f"label {-r - 1} maps outside the block")
+    return out
+
+
+def tpqkld(ao_packed, c):
+    """First half-transform, pq-major H[npq, ilk] (tpqkld.f:25-104)."""
+    c, ldc = _cmat(c)
+    n, m = c.shape
+    v = _f64(ao_packed)
+    h = np.empty((n_pairs(n), n_pairs(m)))
+    lib().or_tpqkld_c1(n, m, _d(v), _d(c), ldc, _d(h))
+    return h
+
+
+def tijkld(h, c):
+    """Second half-transform to the packed MO stream (tijkld.f:28-88)."""
+    c, ldc = _cmat(c)
+    n, m = c.shape
+    h = _f64(h)
+    out = np.empty(n_pairs(n_pairs(m)))
+    lib().or_tijkld_c1(n, m, _d(h), _d(c), ldc, _d(out))
+    return out
+
+
+def ijkl(ao_packed, c):
+    """Whole C1 transform: tpqkld then tijkld."""
+    c, ldc = _cmat(c)
+    n, m = c.shape
+    v = _f64(ao_packed)
+    out = np.empty(n_pairs(n_pairs(m)))
+    if lib().or_ijkl_c1(n, m, _d(v), _d(c), ldc, _d(out)) != 0:
+        raise MemoryError("oracle allocation failed")
+    return out
+
+
+def tmono(h_packed, c):
+    c, ldc = _cmat(c)
+    n, m = c.shape
+    hp = _f64(h_packed)
+    out = np.empty(n_pairs(m))
+    lib().or_tmono(n, m, _d(hp), _d(c), ldc, _d(out))
+    return out
+
+
+def energy(mo_packed, hmo_packed, gamma, Gamma_packed):
+    m = gamma.shape[0]
+    return lib().or_energy(0, m, _d(_f64(mo_packed)), _d(_f64(hmo_packed)), _d(_f64(gamma)),
+                           _d(_f64(Gamma_packed)))
+
+
+def energy_grad(ao_packed, h_packed, c, gamma, Gamma_packed):
+    """E(C) and dE/dC (column-major N x M) in serial FP64."""
+    c, ldc = _cmat(c)
+    n, m = c.shape
+    e = ctypes.c_double()
+    g = np.zeros((n, m), order="F")
+    lib().or_energy_grad(n, m, _d(_f64(ao_packed)), _d(_f64(h_packed)), _d(c), ldc,
+                         _d(_f64(gamma)), _d(_f64(Gamma_packed)), ctypes.byref(e), _d(g))
+    return e.value, g
+
+
+def tpqkld_rows(ao_packed, c, npq_list, nthreads=1):
+    """First half for the listed AO pairs only (rows of H; others left unset)."""
+    c, ldc = _cmat(c)
+    n, m = c.shape
+    v = _f64(ao_packed)
+    lst = np.ascontiguousarray(npq_list, dtype=np.int64)
+    h = np.zeros((n_pairs(n), n_pairs(m)))
+    lib().or_tpqkld_c1_omp_list(n, m, _d(v), _d(c), ldc, _d(h), lst.ctypes.data_as(_lp),
+                                lst.size, nthreads)
+    return h
+
+
+def tijkld_rows(h, c, ilk_list, nthreads=1, out=None):
+    """Second half for the listed MO pairs only."""
+    c, ldc = _cmat(c)
+    n, m = c.shape
+    h = _f64(h)
+    lst = np.ascontiguousarray(ilk_list, dtype=np.int64)
+    if out is None:
+        out = np.zeros(n_pairs(n_pairs(m)))
+    lib().or_tijkld_c1_omp_list(n, m, _d(h), _d(c), ldc, _d(out), lst.ctypes.data_as(_lp),
+                                lst.size, nthreads)
+    return out

@@ -1,0 +1,118 @@
+"""Generate the golden fixtures of the ijkl transform from the reference program itself.
+
+Runs in the survey/build container only (it needs the reference sources under
+/root/reference to build oracle/_ref/ijkl_ref with `make -C oracle`).  For each case it
+writes the reference's own input files (info, pqrs, namelists on stdin), runs the
+flang-built reference ijkl, parses f50 / f25 / ijom and stores the inputs that are
+not regenerable bit-for-bit (C) plus the outputs as .npz fixtures next to this script.
+
+The AO integrals are regenerated from the seed by cipsi_amd.synth (exact by
+construction, so identical on every machine); a SHA-256 of them is stored and
+checked when the fixture is used.
+
+Usage:  python tests/golden/make_golden.py [case ...]
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from cipsi_amd import formats, synth  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ijkl_ref")
+
+# name -> (n_ao, n_mo_active, frozen, rank, decay, seed, cutoff, shuffle, n_samples)
+CASES = {
+    "tiny_n12_frozen2": (12, 10, 2, 30, 4.0, 11, 0.0, False, 0),
+    "sparse_n10_shuffled": (10, 10, 0, 24, 3.0, 12, 0.05, True, 0),
+    "cfg0_n40": (40, 40, 0, 120, 8.0, 2024, 0.0, False, 8192),
+    "cfg1_n120_frozen10": (120, 110, 10, 360, 15.0, 2025, 0.0, False, 8192),
+}
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def run_reference(workdir, n_ao, c_square, h_packed, xx, ix, frozen, enuc=1.25):
+    info = os.path.join(workdir, "info")
+    formats.write_info(info, n_ao, c_square, h_packed, enuc=enuc, n_elec=2)
+    nrec = formats.write_pqrs(os.path.join(workdir, "pqrs"), xx, ix)
+    stdin = formats.ijfil_namelist() + formats.ijklin_namelist(nrec, nvecf=n_ao, frozen=frozen)
+    t0 = time.time()
+    res = subprocess.run([REF_BIN], input=stdin.encode(), cwd=workdir, capture_output=True)
+    dt = time.time() - t0
+    log = res.stdout.decode(errors="replace") + res.stderr.decode(errors="replace")
+    if res.returncode != 0 or "fin ijkl" not in log:
+        raise RuntimeError("reference ijkl failed:\n" + log[-3000:])
+    return nrec, dt, log
+
+
+def make_case(name):
+    n_ao, n_mo, frozen, rank, decay, seed, cutoff, shuffle, n_samples = CASES[name]
+    case = synth.make_case(n_ao, n_mo, rank, decay, seed, frozen=frozen)
+    V = case["ao_packed"]
+    c_full = case["c_full"]
+    h_packed = synth.symmetric_packed(n_ao, seed + 7)
+    xx, ix = formats.labels_from_packed(V, n_ao, cutoff=cutoff)
+    if shuffle:
+        perm = np.random.default_rng(seed + 3).permutation(xx.size)
+        xx, ix = xx[perm], ix[perm]
+    with tempfile.TemporaryDirectory(prefix="ijkl_ref_") as wd:
+        nrec, dt, log = run_reference(wd, n_ao, c_full, h_packed, xx, ix, frozen)
+        m_ref = n_ao                   # nvec = nvecf = N: frozen columns compacted, stale tail kept
+        f50 = formats.read_f50(os.path.join(wd, "f50"), m_ref)
+        f25 = formats.read_f25(os.path.join(wd, "f25"))
+        ijom = formats.read_records(os.path.join(wd, "ijom"))
+        with open(os.path.join(wd, "f25"), "rb") as f:
+            f25_bytes = f.read()
+    pm = n_mo * (n_mo + 1) // 2
+    n_active = pm * (pm + 1) // 2
+    mo = f50["mo"]
+    active = mo[:n_active]                 # leading prefix = transform of the active columns
+    out = dict(
+        n_ao=n_ao, n_mo=n_mo, frozen=frozen, rank=rank, decay=decay, seed=seed, cutoff=cutoff,
+        shuffle=shuffle, nrec=nrec, ref_seconds=dt,
+        c_full=c_full, h_packed=h_packed, ao_sha=sha(V), labels_sha=sha(ix), xx_sha=sha(xx),
+        hmo=f50["hmo"], n_records=f50["n_records"], n_mo_records=f50["n_mo_records"],
+        tail=f50["tail"], f25_bytes=np.frombuffer(f25_bytes, dtype=np.uint8),
+        ijom_c=np.frombuffer(ijom[0], "<f8"), ijom_s=np.frombuffer(ijom[2], "<f8"),
+        full_sum=np.sum(mo), full_sumsq=np.sum(mo * mo), active_sum=np.sum(active),
+        active_sumsq=np.sum(active * active), active_absmax=np.max(np.abs(active)),
+    )
+    if n_samples == 0:
+        out["mo"] = mo
+    else:
+        rng = np.random.default_rng(seed + 99)
+        idx = np.unique(np.concatenate([
+            rng.integers(0, n_active, size=n_samples),
+            np.arange(min(512, n_active)), np.arange(n_active - min(512, n_active), n_active)]))
+        out["sample_idx"] = idx.astype(np.int64)
+        out["sample_val"] = active[idx]
+        # all records' boundaries around the stale tail, full last record
+        out["last_record"] = mo[(f50["n_mo_records"] - 1) * formats.LSIZE:]
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: NREC={nrec} ref {dt:.1f}s f50 records={f50['n_records']} -> {path} "
+          f"({os.path.getsize(path) / 1024:.0f} KiB)")
+
+
+def main(argv):
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"], check=True)
+    names = argv or list(CASES)
+    for n in names:
+        make_case(n)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
