@@ -1,0 +1,533 @@
+// C ABI of the ijkl transform library (declarations: include/cipsi_ijkl.h).
+//
+// The pipeline for one C1 transform (all on one HIP stream, device-resident):
+//   pad_c        C -> zero-padded row-major copy used by the MFMA tiles
+//   expand       packed AO block -> full symmetric pair matrix (rows contiguous)
+//   half 1       per AO pair: C^T X C lower triangle   (tpqkld.f)      -> H[P_A][P_M]
+//   transpose    H -> H^T (MO-pair rows)               (replaces unit 40, itijkl.f:249-278)
+//   half 2       per MO pair: C^T Y C, ij <= kl        (tijkld.f)      -> packed MO stream
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/cipsi_ijkl.h"
+#include "common.h"
+#include "ctx.h"
+
+using namespace cipsi;
+
+// --------------------------------------------------------------------------- helpers
+static int map_hip(cipsi_ijkl_ctx* ctx, hipError_t e, const char* where) {
+    if (e == hipSuccess) return CIPSI_OK;
+    if (ctx) {
+        ctx->last_error = std::string(where) + ": " + hipGetErrorString(e);
+    }
+    (void)hipGetLastError();
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return CIPSI_ERR_NOMEM;
+    return CIPSI_ERR_HIP;
+}
+
+#define HIP_TRY(ctx, expr)                                          \
+    do {                                                            \
+        hipError_t _e = (expr);                                     \
+        if (_e != hipSuccess) return map_hip((ctx), _e, #expr);     \
+    } while (0)
+
+int ctx_alloc(cipsi_ijkl_ctx* ctx, DevBuf& b, size_t bytes) {
+    if (b.bytes >= bytes && b.ptr) return CIPSI_OK;
+    if (b.ptr) { (void)hipFree(b.ptr); b.ptr = nullptr; b.bytes = 0; }
+    if (bytes == 0) return CIPSI_OK;
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) { b.ptr = nullptr; return map_hip(ctx, e, "hipMalloc"); }
+    b.bytes = bytes;
+    return CIPSI_OK;
+}
+
+static void ctx_free(DevBuf& b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+int ctx_enter(cipsi_ijkl_ctx* ctx) {
+    if (!ctx) return CIPSI_ERR_ARG;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return map_hip(ctx, e, "hipSetDevice");
+    return CIPSI_OK;
+}
+
+static int check_dims(int n_ao, int n_mo, int ldc) {
+    if (n_ao < 1 || n_mo < 1 || n_mo > n_ao || ldc < n_ao) return CIPSI_ERR_ARG;
+    if (n_ao > 256) return CIPSI_ERR_UNSUPPORTED;
+    return CIPSI_OK;
+}
+
+static inline int round16(int x) { return (x + 15) & ~15; }
+
+// Half-2 work table: tile_start[l] = sum_{l'<l} (l'+1) * ceil((l'+1)/lt).
+int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int64_t* total) {
+    if (ctx->ts_m != m || ctx->ts_lt != lt) {
+        std::vector<int64_t> t(m + 1);
+        t[0] = 0;
+        for (int l = 0; l < m; ++l) t[l + 1] = t[l] + (int64_t)(l + 1) * ((l + lt) / lt);
+        int rc = ctx_alloc(ctx, ctx->tile_start, sizeof(int64_t) * (m + 1));
+        if (rc) return rc;
+        HIP_TRY(ctx, hipMemcpy(ctx->tile_start.ptr, t.data(), sizeof(int64_t) * (m + 1),
+                               hipMemcpyHostToDevice));
+        ctx->ts_m = m;
+        ctx->ts_lt = lt;
+        ctx->ts_total = t[m];
+    }
+    *out = (const int64_t*)ctx->tile_start.ptr;
+    *total = ctx->ts_total;
+    return CIPSI_OK;
+}
+
+int ctx_half_lt(const cipsi_ijkl_ctx* ctx, int half, int n, int m) {
+    (void)n; (void)m;
+    return half == 1 ? ctx->lt_half1 : ctx->lt_half2;
+}
+
+// --------------------------------------------------------------------------- pipeline pieces
+int run_pad_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_c, int ldc, double** cr) {
+    const int np = round16(n), mp = round16(m);
+    int rc = ctx_alloc(ctx, ctx->cr, sizeof(double) * (size_t)np * mp);
+    if (rc) return rc;
+    StageTimer tm(ctx, "pad_c");
+    HIP_TRY(ctx, launch_pad_c(d_c, ldc, n, m, np, mp, (double*)ctx->cr.ptr, ctx->stream));
+    *cr = (double*)ctx->cr.ptr;
+    return CIPSI_OK;
+}
+
+int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, const double* cr,
+             double* dst, int precision, int n_slices) {
+    const int64_t pa = npairs(n), pm = npairs(m);
+    HalfArgs a{};
+    a.src = src;
+    a.src_ld = pa;
+    a.cr = cr;
+    a.ldm = round16(m);
+    a.dst = dst;
+    a.n = n;
+    a.m = m;
+    a.np = round16(n);
+    a.mp = round16(m);
+    a.half = half;
+    a.b_begin = 0;
+    const int lt = ctx_half_lt(ctx, half, n, m);
+    if (half == 1) {
+        a.nb = pa;
+        a.dst_ld = pm;
+        a.ntile_total = pa * (int64_t)((m + lt - 1) / lt);
+        a.tile_start = nullptr;
+    } else {
+        a.nb = pm;
+        a.dst_ld = 0;
+        const int64_t* ts;
+        int64_t total;
+        int rc = ctx_tile_table(ctx, m, lt, &ts, &total);
+        if (rc) return rc;
+        a.tile_start = ts;
+        a.ntile_total = total;
+    }
+    StageTimer tm(ctx, half == 1 ? "half1" : "half2");
+    if (precision == CIPSI_PREC_FP64) {
+        HIP_TRY(ctx, launch_half_fp64(a, lt, ctx->stream));
+        return CIPSI_OK;
+    }
+    if (precision == CIPSI_PREC_SPLIT_FP16) {
+        return run_half_split(ctx, a, lt, n_slices);
+    }
+    return CIPSI_ERR_ARG;
+}
+
+static int check_prec(int precision, int n_slices) {
+    if (precision == CIPSI_PREC_FP64) return CIPSI_OK;
+    if (precision == CIPSI_PREC_SPLIT_FP16) return split_supported(n_slices) ? CIPSI_OK : CIPSI_ERR_UNSUPPORTED;
+    return CIPSI_ERR_ARG;
+}
+
+// --------------------------------------------------------------------------- ABI
+extern "C" {
+
+const char* cipsi_ijkl_strerror(int s) {
+    switch (s) {
+        case CIPSI_OK: return "ok";
+        case CIPSI_ERR_ARG: return "invalid argument (dimension, pointer or leading dimension)";
+        case CIPSI_ERR_LABEL: return "label stream: N_AO > 255 or a label outside the AO block";
+        case CIPSI_ERR_NOMEM: return "device memory allocation failed";
+        case CIPSI_ERR_HIP: return "HIP runtime error";
+        case CIPSI_ERR_NONFINITE: return "non-finite value in an input";
+        case CIPSI_ERR_NOT_ORTHONORMAL: return "MO coefficients are not orthonormal";
+        case CIPSI_ERR_UNSUPPORTED: return "unsupported size, precision or slice count";
+        case CIPSI_ERR_NODEVICE: return "no usable gfx950 device";
+        default: return "unknown status";
+    }
+}
+
+const char* cipsi_ijkl_last_error(const cipsi_ijkl_ctx* ctx) {
+    return ctx ? ctx->last_error.c_str() : "";
+}
+
+int cipsi_ijkl_create(cipsi_ijkl_ctx** out, int device) {
+    if (!out) return CIPSI_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        (void)hipGetLastError();
+        return CIPSI_ERR_NODEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return CIPSI_ERR_NODEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return CIPSI_ERR_NODEVICE;
+    cipsi_ijkl_ctx* ctx = new (std::nothrow) cipsi_ijkl_ctx();
+    if (!ctx) return CIPSI_ERR_NOMEM;
+    ctx->device = device;
+    ctx->num_cu = prop.multiProcessorCount;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return CIPSI_ERR_HIP;
+    }
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return CIPSI_OK;
+}
+
+int cipsi_ijkl_release_workspace(cipsi_ijkl_ctx* ctx) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx_free(ctx->vsq);
+    ctx_free(ctx->h);
+    ctx_free(ctx->cr);
+    ctx_free(ctx->packed);
+    ctx_free(ctx->owner);
+    ctx_free(ctx->io_in);
+    ctx_free(ctx->io_in2);
+    ctx_free(ctx->io_out);
+    ctx_free(ctx->io_c);
+    ctx_free(ctx->scal);
+    ctx_free(ctx->tile_start);
+    ctx_free(ctx->slices);
+    ctx->ts_m = -1;
+    return CIPSI_OK;
+}
+
+int cipsi_ijkl_destroy(cipsi_ijkl_ctx* ctx) {
+    if (!ctx) return CIPSI_ERR_ARG;
+    cipsi_ijkl_reset_timing(ctx);
+    cipsi_ijkl_release_workspace(ctx);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return CIPSI_OK;
+}
+
+int cipsi_ijkl_set_stream(cipsi_ijkl_ctx* ctx, void* s) {
+    if (!ctx) return CIPSI_ERR_ARG;
+    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    return CIPSI_OK;
+}
+
+int cipsi_ijkl_synchronize(cipsi_ijkl_ctx* ctx) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CIPSI_OK;
+}
+
+int64_t cipsi_ijkl_workspace_bytes(int n_ao, int n_mo, int precision, int with_labels) {
+    if (check_dims(n_ao, n_mo, n_ao)) return -1;
+    const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
+    int64_t b = 8 * (pa * pa + pa * pm) + 8 * (int64_t)round16(n_ao) * round16(n_mo);
+    if (with_labels) b += 12 * npairs(pa);
+    if (precision == CIPSI_PREC_SPLIT_FP16) b += split_workspace_bytes(n_ao, n_mo);
+    return b;
+}
+
+int cipsi_ijkl_transform_packed_dev(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo,
+                                    const double* d_ao, const double* d_c, int ldc, double* d_mo,
+                                    int precision, int n_slices) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_dims(n_ao, n_mo, ldc))) return rc;
+    if ((rc = check_prec(precision, n_slices))) return rc;
+    if (!d_ao || !d_c || !d_mo) return CIPSI_ERR_ARG;
+    const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
+    if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->h, sizeof(double) * pa * pm))) return rc;
+    double* cr;
+    if ((rc = run_pad_c(ctx, n_ao, n_mo, d_c, ldc, &cr))) return rc;
+    double* vsq = (double*)ctx->vsq.ptr;
+    double* h = (double*)ctx->h.ptr;
+    {
+        StageTimer tm(ctx, "expand");
+        HIP_TRY(ctx, launch_expand_packed(d_ao, pa, vsq, ctx->stream));
+    }
+    if ((rc = run_half(ctx, 1, n_ao, n_mo, vsq, cr, h, precision, n_slices))) return rc;
+    {
+        StageTimer tm(ctx, "transpose");
+        HIP_TRY(ctx, launch_transpose(h, pa, pm, vsq, ctx->stream));   // H^T reuses vsq
+    }
+    if ((rc = run_half(ctx, 2, n_ao, n_mo, vsq, cr, d_mo, precision, n_slices))) return rc;
+    return CIPSI_OK;
+}
+
+int cipsi_ijkl_unpack_labels_dev(cipsi_ijkl_ctx* ctx, int n_ao, const double* d_xx,
+                                 const int32_t* d_labels, int64_t n_entries, double* d_ao) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if (n_ao < 1 || n_entries < 0 || !d_ao || (n_entries > 0 && (!d_xx || !d_labels)))
+        return CIPSI_ERR_ARG;
+    if (n_ao > 255) return CIPSI_ERR_LABEL;
+    if (n_entries >= (int64_t)0xffffffffLL) return CIPSI_ERR_ARG;
+    const int64_t nblk = npairs(npairs(n_ao));
+    if ((rc = ctx_alloc(ctx, ctx->owner, sizeof(uint32_t) * nblk))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->scal, 64))) return rc;
+    int64_t* first_zero = (int64_t*)ctx->scal.ptr;
+    int* err_flag = (int*)((char*)ctx->scal.ptr + 16);
+    {
+        StageTimer tm(ctx, "unpack");
+        HIP_TRY(ctx, launch_unpack_labels(d_xx, d_labels, n_entries, n_ao, d_ao,
+                                          (uint32_t*)ctx->owner.ptr, first_zero, err_flag,
+                                          ctx->stream));
+    }
+    int herr = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&herr, err_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return herr ? CIPSI_ERR_LABEL : CIPSI_OK;
+}
+
+int cipsi_ijkl_transform_labels_dev(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* d_xx,
+                                    const int32_t* d_labels, int64_t n_entries, const double* d_c,
+                                    int ldc, double* d_mo, int precision, int n_slices) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_dims(n_ao, n_mo, ldc))) return rc;
+    if ((rc = check_prec(precision, n_slices))) return rc;
+    const int64_t nblk = npairs(npairs(n_ao));
+    if ((rc = ctx_alloc(ctx, ctx->packed, sizeof(double) * nblk))) return rc;
+    if ((rc = cipsi_ijkl_unpack_labels_dev(ctx, n_ao, d_xx, d_labels, n_entries,
+                                           (double*)ctx->packed.ptr)))
+        return rc;
+    return cipsi_ijkl_transform_packed_dev(ctx, n_ao, n_mo, (const double*)ctx->packed.ptr, d_c,
+                                           ldc, d_mo, precision, n_slices);
+}
+
+int cipsi_ijkl_half1_dev(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* d_ao,
+                         const double* d_c, int ldc, double* d_h, int precision, int n_slices) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_dims(n_ao, n_mo, ldc))) return rc;
+    if ((rc = check_prec(precision, n_slices))) return rc;
+    if (!d_ao || !d_c || !d_h) return CIPSI_ERR_ARG;
+    const int64_t pa = npairs(n_ao);
+    if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
+    double* cr;
+    if ((rc = run_pad_c(ctx, n_ao, n_mo, d_c, ldc, &cr))) return rc;
+    {
+        StageTimer tm(ctx, "expand");
+        HIP_TRY(ctx, launch_expand_packed(d_ao, pa, (double*)ctx->vsq.ptr, ctx->stream));
+    }
+    return run_half(ctx, 1, n_ao, n_mo, (const double*)ctx->vsq.ptr, cr, d_h, precision, n_slices);
+}
+
+int cipsi_ijkl_half2_dev(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* d_h,
+                         const double* d_c, int ldc, double* d_mo, int precision, int n_slices) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_dims(n_ao, n_mo, ldc))) return rc;
+    if ((rc = check_prec(precision, n_slices))) return rc;
+    if (!d_h || !d_c || !d_mo) return CIPSI_ERR_ARG;
+    const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
+    if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pm))) return rc;
+    double* cr;
+    if ((rc = run_pad_c(ctx, n_ao, n_mo, d_c, ldc, &cr))) return rc;
+    {
+        StageTimer tm(ctx, "transpose");
+        HIP_TRY(ctx, launch_transpose(d_h, pa, pm, (double*)ctx->vsq.ptr, ctx->stream));
+    }
+    return run_half(ctx, 2, n_ao, n_mo, (const double*)ctx->vsq.ptr, cr, d_mo, precision, n_slices);
+}
+
+// ------------------------------------------------------------------ host-pointer entry points
+static int check_finite_host(const double* p, int64_t n) {
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return CIPSI_ERR_NONFINITE;
+    return CIPSI_OK;
+}
+
+static int upload_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* c, int ldc, double** d_c) {
+    int rc = ctx_alloc(ctx, ctx->io_c, sizeof(double) * (size_t)ldc * m);
+    if (rc) return rc;
+    for (int j = 0; j < m; ++j)
+        if ((rc = check_finite_host(c + (int64_t)j * ldc, n))) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->io_c.ptr, c, sizeof(double) * (size_t)ldc * m,
+                                hipMemcpyHostToDevice, ctx->stream));
+    *d_c = (double*)ctx->io_c.ptr;
+    return CIPSI_OK;
+}
+
+double err_estimate(int n, int precision, int n_slices, double vmax);
+
+int cipsi_ijkl_transform_packed(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* ao,
+                                const double* c, int ldc, double* mo, int precision, int n_slices,
+                                double* err_est) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_dims(n_ao, n_mo, ldc))) return rc;
+    if ((rc = check_prec(precision, n_slices))) return rc;
+    if (!ao || !c || !mo) return CIPSI_ERR_ARG;
+    const int64_t nin = npairs(npairs(n_ao)), nout = npairs(npairs(n_mo));
+    double vmax = 0.0;
+    for (int64_t i = 0; i < nin; ++i) {
+        if (!std::isfinite(ao[i])) return CIPSI_ERR_NONFINITE;
+        vmax = std::fmax(vmax, std::fabs(ao[i]));
+    }
+    if ((rc = ctx_alloc(ctx, ctx->io_in, sizeof(double) * nin))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->io_out, sizeof(double) * nout))) return rc;
+    double* d_c;
+    if ((rc = upload_c(ctx, n_ao, n_mo, c, ldc, &d_c))) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->io_in.ptr, ao, sizeof(double) * nin, hipMemcpyHostToDevice,
+                                ctx->stream));
+    if ((rc = cipsi_ijkl_transform_packed_dev(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr, d_c,
+                                              ldc, (double*)ctx->io_out.ptr, precision, n_slices)))
+        return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(mo, ctx->io_out.ptr, sizeof(double) * nout, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (err_est) *err_est = err_estimate(n_ao, precision, n_slices, vmax);
+    return CIPSI_OK;
+}
+
+int cipsi_ijkl_transform_labels(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* xx,
+                                const int32_t* labels, int64_t n_entries, const double* c, int ldc,
+                                double* mo, int precision, int n_slices, double* err_est) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_dims(n_ao, n_mo, ldc))) return rc;
+    if (n_ao > 255) return CIPSI_ERR_LABEL;
+    if ((rc = check_prec(precision, n_slices))) return rc;
+    if (n_entries < 0 || !c || !mo || (n_entries > 0 && (!xx || !labels))) return CIPSI_ERR_ARG;
+    double vmax = 0.0;
+    for (int64_t i = 0; i < n_entries; ++i) {
+        if (labels[i] == 0) break;
+        if (!std::isfinite(xx[i])) return CIPSI_ERR_NONFINITE;
+        vmax = std::fmax(vmax, 8.0 * std::fabs(xx[i]));
+    }
+    const int64_t nout = npairs(npairs(n_mo));
+    const int64_t ne = n_entries > 0 ? n_entries : 1;
+    if ((rc = ctx_alloc(ctx, ctx->io_in, sizeof(double) * ne))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->io_in2, sizeof(int32_t) * ne))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->io_out, sizeof(double) * nout))) return rc;
+    double* d_c;
+    if ((rc = upload_c(ctx, n_ao, n_mo, c, ldc, &d_c))) return rc;
+    if (n_entries > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->io_in.ptr, xx, sizeof(double) * n_entries,
+                                    hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->io_in2.ptr, labels, sizeof(int32_t) * n_entries,
+                                    hipMemcpyHostToDevice, ctx->stream));
+    }
+    if ((rc = cipsi_ijkl_transform_labels_dev(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr,
+                                              (const int32_t*)ctx->io_in2.ptr, n_entries, d_c, ldc,
+                                              (double*)ctx->io_out.ptr, precision, n_slices)))
+        return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(mo, ctx->io_out.ptr, sizeof(double) * nout, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (err_est) *err_est = err_estimate(n_ao, precision, n_slices, vmax);
+    return CIPSI_OK;
+}
+
+int cipsi_ijkl_tmono(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* h_packed,
+                     const double* c, int ldc, double* h_mo) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_dims(n_ao, n_mo, ldc))) return rc;
+    if (!h_packed || !c || !h_mo) return CIPSI_ERR_ARG;
+    const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
+    if ((rc = check_finite_host(h_packed, pa))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->io_in, sizeof(double) * pa))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->io_out, sizeof(double) * pm))) return rc;
+    double* d_c;
+    if ((rc = upload_c(ctx, n_ao, n_mo, c, ldc, &d_c))) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->io_in.ptr, h_packed, sizeof(double) * pa,
+                                hipMemcpyHostToDevice, ctx->stream));
+    double* cr;
+    if ((rc = run_pad_c(ctx, n_ao, n_mo, d_c, ldc, &cr))) return rc;
+    HalfArgs a{};
+    a.src = (const double*)ctx->io_in.ptr;
+    a.src_ld = pa;
+    a.cr = cr;
+    a.ldm = round16(n_mo);
+    a.dst = (double*)ctx->io_out.ptr;
+    a.n = n_ao;
+    a.m = n_mo;
+    a.np = round16(n_ao);
+    a.mp = round16(n_mo);
+    a.half = 1;
+    a.nb = 1;
+    a.dst_ld = pm;
+    a.ntile_total = (n_mo + 31) / 32;
+    HIP_TRY(ctx, launch_half_fp64(a, 32, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(h_mo, ctx->io_out.ptr, sizeof(double) * pm, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CIPSI_OK;
+}
+
+static void drain_timing(cipsi_ijkl_ctx* ctx) {
+    for (auto& p : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto& t = ctx->totals[p.stage];
+            t.first += ms;
+            t.second += 1;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    ctx->pending.clear();
+}
+
+int cipsi_ijkl_enable_timing(cipsi_ijkl_ctx* ctx, int enable) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    ctx->timing = enable != 0;
+    return CIPSI_OK;
+}
+
+int cipsi_ijkl_reset_timing(cipsi_ijkl_ctx* ctx) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    drain_timing(ctx);
+    ctx->totals.clear();
+    return CIPSI_OK;
+}
+
+int cipsi_ijkl_read_timing(cipsi_ijkl_ctx* ctx, const char* stage, double* total_ms,
+                           int64_t* launches) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if (!stage || !total_ms || !launches) return CIPSI_ERR_ARG;
+    drain_timing(ctx);
+    auto it = ctx->totals.find(stage);
+    *total_ms = it == ctx->totals.end() ? 0.0 : it->second.first;
+    *launches = it == ctx->totals.end() ? 0 : it->second.second;
+    return CIPSI_OK;
+}
+
+}  // extern "C"
+
+// A-priori error estimate: four chained length-N dot products with orthonormal C
+// (column 2-norms 1), unit roundoff u of the arithmetic, random-walk growth sqrt(N).
+double err_estimate(int n, int precision, int n_slices, double vmax) {
+    double u = std::ldexp(1.0, -53);
+    if (precision == CIPSI_PREC_SPLIT_FP16) u = split_unit_roundoff(n, n_slices);
+    return 4.0 * std::sqrt((double)n) * u * vmax;
+}

@@ -1,0 +1,69 @@
+// Shared definitions of the MI355X (gfx950) ijkl transform library.
+//
+// Index conventions (0-based; the reference is 1-based):
+//   pair0(a, b) = a(a+1)/2 + b, a >= b         (CIPSI_sources/ijkl/itijkl.f:38)
+//   P_A = N(N+1)/2 AO pairs, P_M = M(M+1)/2 MO pairs
+//   packed AO block   V[pair0(ij, kl)], ij >= kl            (transd.f:369-371)
+//   half-transformed  H[pq * P_M + pair0(l, k)], l >= k      (tpqkld.f:91-102)
+//   packed MO stream  out[pair0(kl, pair0(j, i))], ij <= kl   (tijkld.f:159-186)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cipsi {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__host__ __device__ inline int64_t pair0(int64_t a, int64_t b) { return a * (a + 1) / 2 + b; }
+__host__ __device__ inline int64_t pairsym(int64_t a, int64_t b) { return a >= b ? pair0(a, b) : pair0(b, a); }
+__host__ __device__ inline int64_t npairs(int64_t n) { return n * (n + 1) / 2; }
+
+// Largest a with pair0(a, 0) <= x (row of packed index x).
+__device__ inline int64_t pair_row(int64_t x) {
+    int64_t a = (int64_t)((sqrt(8.0 * (double)x + 1.0) - 1.0) * 0.5);
+    while (pair0(a, 0) > x) --a;
+    while (pair0(a + 1, 0) <= x) ++a;
+    return a;
+}
+
+// One half of the transform, as a batch of "sandwiches": for every matrix index b
+//   Z_b(r, s) = src[b * src_ld + pair0(max(r,s), min(r,s))]  (symmetric n x n)
+//   O_b = C^T Z_b C  restricted to a <= c and pair0(c, a) < limit_b, written to
+//   dst[dst_off_b + pair0(c, a)].
+// Half 1 (tpqkld): b = AO pair pq, src = full symmetric AO supermatrix rows,
+//   limit = P_M, dst_off = pq * P_M.
+// Half 2 (tijkld): b = MO pair kl, src = H^T rows, limit = kl + 1,
+//   dst_off = pair0(kl, 0); only columns c <= l (kl = pair0(l, k)) are needed.
+struct HalfArgs {
+    const double* src;     // rows of packed symmetric matrices
+    int64_t src_ld;        // = P_A
+    const double* cr;      // C row-major, zero padded: cr[r * ldm + a], r < np, a < mp
+    int ldm;               // = mp
+    double* dst;
+    int n;                 // N_AO
+    int m;                 // N_MO
+    int np;                // N rounded up to 16
+    int mp;                // M rounded up to 16
+    int half;              // 1 or 2
+    int64_t nb;            // number of matrices (P_A or P_M)
+    int64_t b_begin;       // first matrix index handled by this launch
+    int64_t dst_ld;        // half 1: P_M
+    int64_t ntile_total;   // number of (b, column tile) work items
+    const int64_t* tile_start;  // half 2: prefix table over l (m + 1 entries), device
+};
+
+// Device launchers (defined in the .hip files).
+hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t n_entries,
+                                int n_ao, double* ao_packed, uint32_t* owner, int64_t* first_zero,
+                                int* err_flag, hipStream_t st);
+hipError_t launch_expand_packed(const double* ao_packed, int64_t pa, double* vsq, hipStream_t st);
+hipError_t launch_transpose(const double* src, int64_t rows, int64_t cols, double* dst,
+                            hipStream_t st);
+hipError_t launch_pad_c(const double* c, int ldc, int n, int m, int np, int mp, double* cr,
+                        hipStream_t st);
+hipError_t launch_half_fp64(const HalfArgs& a, int lt, hipStream_t st);
+hipError_t launch_zero(double* p, int64_t n, hipStream_t st);
+
+}  // namespace cipsi

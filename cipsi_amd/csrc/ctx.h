@@ -1,0 +1,73 @@
+// Library context (opaque to callers of include/cipsi_ijkl.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct cipsi_ijkl_ctx {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    // workspace (grow-only, device)
+    DevBuf vsq;      // full symmetric AO pair matrix, later H^T
+    DevBuf h;        // H[P_A][P_M]
+    DevBuf cr;       // padded row-major C
+    DevBuf packed;   // packed AO block built from labels
+    DevBuf owner;    // last-writer election for duplicate labels
+    DevBuf scal;     // small device scalars (first zero label, error flag)
+    DevBuf io_in, io_in2, io_out, io_c;   // staging for host-pointer entry points
+    DevBuf tile_start;                    // half-2 work table
+    DevBuf slices;   // split-FP16 operand slices
+    int ts_m = -1, ts_lt = -1;
+    int64_t ts_total = 0;
+    // tile plans (columns per work-group) for the two halves
+    int lt_half1 = 32;
+    int lt_half2 = 32;
+    // per-stage event timing
+    bool timing = false;
+    struct Pending { std::string stage; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::map<std::string, std::pair<double, int64_t>> totals;
+};
+
+// Records HIP events around one stage launch on the context stream when timing is on.
+struct StageTimer {
+    cipsi_ijkl_ctx* ctx;
+    const char* stage;
+    hipEvent_t a = nullptr;
+    StageTimer(cipsi_ijkl_ctx* c, const char* s) : ctx(c), stage(s) {
+        if (ctx->timing && hipEventCreate(&a) == hipSuccess) (void)hipEventRecord(a, ctx->stream);
+        else a = nullptr;
+    }
+    ~StageTimer() {
+        if (!a) return;
+        hipEvent_t b;
+        if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->pending.push_back({stage, a, b});
+    }
+};
+
+int ctx_alloc(cipsi_ijkl_ctx* ctx, DevBuf& b, size_t bytes);
+int ctx_enter(cipsi_ijkl_ctx* ctx);
+int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int64_t* total);
+int run_pad_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_c, int ldc, double** cr);
+int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, const double* cr,
+             double* dst, int precision, int n_slices);
+
+// split-FP16 path (half_split.hip)
+bool split_supported(int n_slices);
+int64_t split_workspace_bytes(int n_ao, int n_mo);
+double split_unit_roundoff(int n_ao, int n_slices);
+int run_half_split(cipsi_ijkl_ctx* ctx, const cipsi::HalfArgs& a, int lt, int n_slices);

@@ -1,0 +1,278 @@
+"""Python host mirror of the ijkl step, over the C ABI of libcipsi_ijkl.so.
+
+Function names follow the reference routines they replace (CIPSI_sources/ijkl/):
+  transd  -> unpack of the pshf label stream into the packed AO block (transd.f)
+  tpqkld  -> first half-transform, packed AO block -> H[P_A, P_M]        (tpqkld.f)
+  tijkld  -> second half-transform, H -> packed MO stream                (tijkld.f)
+  itijkl  -> the whole transform from the label stream                   (itijkl.f)
+  tmono   -> one-electron transform h_MO = C^T h C                       (tmono.f)
+  transform -> the whole transform from a packed AO block
+
+Arrays may be numpy (host entry points, synchronous) or torch CUDA tensors
+(device entry points, asynchronous on torch's current stream).  Where the
+reference stops with a message (ijkl.f:118-120, 304-316, rotat2.f:347-380) these
+raise IjklError carrying the library status.  There is no CPU fallback: without the
+built HIP library (or a gfx950 device) every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcipsi_ijkl.so")
+
+PREC_FP64 = 0
+PREC_SPLIT_FP16 = 1
+_PREC = {"fp64": PREC_FP64, "split_fp16": PREC_SPLIT_FP16, PREC_FP64: PREC_FP64,
+         PREC_SPLIT_FP16: PREC_SPLIT_FP16}
+
+STATUS = {0: "CIPSI_OK", -1: "CIPSI_ERR_ARG", -2: "CIPSI_ERR_LABEL", -3: "CIPSI_ERR_NOMEM",
+          -4: "CIPSI_ERR_HIP", -5: "CIPSI_ERR_NONFINITE", -6: "CIPSI_ERR_NOT_ORTHONORMAL",
+          -7: "CIPSI_ERR_UNSUPPORTED", -8: "CIPSI_ERR_NODEVICE"}
+
+# every symbol include/cipsi_ijkl.h declares
+EXPORTS = [
+    "cipsi_ijkl_create", "cipsi_ijkl_destroy", "cipsi_ijkl_strerror", "cipsi_ijkl_last_error",
+    "cipsi_ijkl_set_stream", "cipsi_ijkl_synchronize", "cipsi_ijkl_workspace_bytes",
+    "cipsi_ijkl_release_workspace", "cipsi_ijkl_transform_packed", "cipsi_ijkl_transform_labels",
+    "cipsi_ijkl_transform_packed_dev", "cipsi_ijkl_transform_labels_dev",
+    "cipsi_ijkl_unpack_labels_dev", "cipsi_ijkl_half1_dev", "cipsi_ijkl_half2_dev",
+    "cipsi_ijkl_tmono", "cipsi_ijkl_enable_timing", "cipsi_ijkl_read_timing",
+    "cipsi_ijkl_reset_timing",
+]
+
+
+class IjklError(RuntimeError):
+    def __init__(self, status: int, what: str, detail: str = ""):
+        self.status = status
+        msg = f"{what}: {STATUS.get(status, status)}"
+        if detail:
+            msg += f" ({detail})"
+        super().__init__(msg)
+
+
+_lib = None
+_c_int, _c_i64, _vp, _dp = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libcipsi_ijkl.so and declare its prototypes.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() "
+                           "(make -C cipsi_amd/csrc)")
+    L = ctypes.CDLL(path)
+    L.cipsi_ijkl_create.argtypes = [ctypes.POINTER(_vp), _c_int]
+    L.cipsi_ijkl_destroy.argtypes = [_vp]
+    L.cipsi_ijkl_strerror.restype = ctypes.c_char_p
+    L.cipsi_ijkl_strerror.argtypes = [_c_int]
+    L.cipsi_ijkl_last_error.restype = ctypes.c_char_p
+    L.cipsi_ijkl_last_error.argtypes = [_vp]
+    L.cipsi_ijkl_set_stream.argtypes = [_vp, _vp]
+    L.cipsi_ijkl_synchronize.argtypes = [_vp]
+    L.cipsi_ijkl_workspace_bytes.restype = _c_i64
+    L.cipsi_ijkl_workspace_bytes.argtypes = [_c_int, _c_int, _c_int, _c_int]
+    L.cipsi_ijkl_release_workspace.argtypes = [_vp]
+    L.cipsi_ijkl_transform_packed.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int,
+                                              _c_int, _vp]
+    L.cipsi_ijkl_transform_labels.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_int,
+                                              _vp, _c_int, _c_int, _vp]
+    L.cipsi_ijkl_transform_packed_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_int, _vp,
+                                                  _c_int, _c_int]
+    L.cipsi_ijkl_transform_labels_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp,
+                                                  _c_int, _vp, _c_int, _c_int]
+    L.cipsi_ijkl_unpack_labels_dev.argtypes = [_vp, _c_int, _vp, _vp, _c_i64, _vp]
+    L.cipsi_ijkl_half1_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _c_int]
+    L.cipsi_ijkl_half2_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _c_int]
+    L.cipsi_ijkl_tmono.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_int, _vp]
+    L.cipsi_ijkl_enable_timing.argtypes = [_vp, _c_int]
+    L.cipsi_ijkl_read_timing.argtypes = [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(_c_i64)]
+    L.cipsi_ijkl_reset_timing.argtypes = [_vp]
+    _lib = L
+    return L
+
+
+def n_pairs(n: int) -> int:
+    return n * (n + 1) // 2
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(x):
+    if _is_torch(x):
+        return ctypes.c_void_p(x.data_ptr())
+    return ctypes.c_void_p(x.ctypes.data)
+
+
+class Context:
+    """Owns a library context (device workspace, stream, tile plans) on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = _vp()
+        rc = self.lib.cipsi_ijkl_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise IjklError(rc, "cipsi_ijkl_create", self.lib.cipsi_ijkl_strerror(rc).decode())
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.cipsi_ijkl_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --------------------------------------------------------------- plumbing
+    def _check(self, rc, what):
+        if rc != 0:
+            detail = self.lib.cipsi_ijkl_last_error(self.handle).decode(errors="replace")
+            raise IjklError(rc, what, detail or self.lib.cipsi_ijkl_strerror(rc).decode())
+
+    def _use_torch_stream(self):
+        import torch
+        self.lib.cipsi_ijkl_set_stream(self.handle, _vp(torch.cuda.current_stream().cuda_stream))
+
+    def synchronize(self):
+        self._check(self.lib.cipsi_ijkl_synchronize(self.handle), "cipsi_ijkl_synchronize")
+
+    def release_workspace(self):
+        self._check(self.lib.cipsi_ijkl_release_workspace(self.handle), "release_workspace")
+
+    @staticmethod
+    def _cmat(c):
+        """(array, ldc): C as column-major N x M with leading dimension N."""
+        if _is_torch(c):
+            import torch
+            if c.dtype != torch.float64 or not c.is_cuda:
+                raise TypeError("C must be a float64 CUDA tensor")
+            ct = c.t().contiguous()            # (M, N) row-major == (N, M) column-major
+            return ct, c.shape[0]
+        c = np.asfortranarray(c, dtype=np.float64)
+        return c, c.shape[0]
+
+    # --------------------------------------------------------------- operators
+    def transform(self, ao_packed, c, precision="fp64", n_slices=0, return_err=False):
+        """Packed AO block -> packed MO stream (the whole C1 itijkl path)."""
+        prec = _PREC[precision]
+        cm, ldc = self._cmat(c)
+        n, m = c.shape
+        nout = n_pairs(n_pairs(m))
+        if _is_torch(ao_packed):
+            import torch
+            if ao_packed.numel() != n_pairs(n_pairs(n)):
+                raise ValueError("ao_packed has the wrong length")
+            out = torch.empty(nout, dtype=torch.float64, device=ao_packed.device)
+            self._use_torch_stream()
+            self._check(self.lib.cipsi_ijkl_transform_packed_dev(
+                self.handle, n, m, _ptr(ao_packed), _ptr(cm), ldc, _ptr(out), prec, n_slices),
+                "transform_packed_dev")
+            return out
+        ao = np.ascontiguousarray(ao_packed, dtype=np.float64)
+        if ao.size != n_pairs(n_pairs(n)):
+            raise ValueError("ao_packed has the wrong length")
+        out = np.empty(nout)
+        err = ctypes.c_double(0.0)
+        self.lib.cipsi_ijkl_set_stream(self.handle, None)
+        self._check(self.lib.cipsi_ijkl_transform_packed(
+            self.handle, n, m, _ptr(ao), _ptr(cm), ldc, _ptr(out), prec, n_slices,
+            ctypes.byref(err)), "transform_packed")
+        return (out, err.value) if return_err else out
+
+    def itijkl(self, xx, ix, c, precision="fp64", n_slices=0, return_err=False):
+        """pshf label stream (values, 32-bit labels) -> packed MO stream."""
+        prec = _PREC[precision]
+        cm, ldc = self._cmat(c)
+        n, m = c.shape
+        nout = n_pairs(n_pairs(m))
+        if _is_torch(xx):
+            import torch
+            out = torch.empty(nout, dtype=torch.float64, device=xx.device)
+            self._use_torch_stream()
+            self._check(self.lib.cipsi_ijkl_transform_labels_dev(
+                self.handle, n, m, _ptr(xx), _ptr(ix), xx.numel(), _ptr(cm), ldc, _ptr(out),
+                prec, n_slices), "transform_labels_dev")
+            return out
+        xx = np.ascontiguousarray(xx, dtype=np.float64)
+        ix = np.ascontiguousarray(ix, dtype=np.int32)
+        out = np.empty(nout)
+        err = ctypes.c_double(0.0)
+        self.lib.cipsi_ijkl_set_stream(self.handle, None)
+        self._check(self.lib.cipsi_ijkl_transform_labels(
+            self.handle, n, m, _ptr(xx), _ptr(ix), xx.size, _ptr(cm), ldc, _ptr(out), prec,
+            n_slices, ctypes.byref(err)), "transform_labels")
+        return (out, err.value) if return_err else out
+
+    def transd(self, xx, ix, n_ao):
+        """Label stream -> packed AO block (device tensors)."""
+        import torch
+        out = torch.empty(n_pairs(n_pairs(n_ao)), dtype=torch.float64, device=xx.device)
+        self._use_torch_stream()
+        self._check(self.lib.cipsi_ijkl_unpack_labels_dev(
+            self.handle, n_ao, _ptr(xx), _ptr(ix), xx.numel(), _ptr(out)), "unpack_labels_dev")
+        return out
+
+    def tpqkld(self, ao_packed, c, precision="fp64", n_slices=0):
+        """First half-transform on device tensors: H[P_A, P_M]."""
+        import torch
+        cm, ldc = self._cmat(c)
+        n, m = c.shape
+        h = torch.empty((n_pairs(n), n_pairs(m)), dtype=torch.float64, device=ao_packed.device)
+        self._use_torch_stream()
+        self._check(self.lib.cipsi_ijkl_half1_dev(
+            self.handle, n, m, _ptr(ao_packed), _ptr(cm), ldc, _ptr(h), _PREC[precision],
+            n_slices), "half1_dev")
+        return h
+
+    def tijkld(self, h, c, precision="fp64", n_slices=0):
+        """Second half-transform on device tensors: H -> packed MO stream."""
+        import torch
+        cm, ldc = self._cmat(c)
+        n, m = c.shape
+        h = h.contiguous()
+        out = torch.empty(n_pairs(n_pairs(m)), dtype=torch.float64, device=h.device)
+        self._use_torch_stream()
+        self._check(self.lib.cipsi_ijkl_half2_dev(
+            self.handle, n, m, _ptr(h), _ptr(cm), ldc, _ptr(out), _PREC[precision], n_slices),
+            "half2_dev")
+        return out
+
+    # --------------------------------------------------------------- timing
+    def enable_timing(self, on=True):
+        self._check(self.lib.cipsi_ijkl_enable_timing(self.handle, int(on)), "enable_timing")
+
+    def reset_timing(self):
+        self._check(self.lib.cipsi_ijkl_reset_timing(self.handle), "reset_timing")
+
+    def read_timing(self, stage):
+        """(total milliseconds, launches) of one pipeline stage since the last reset."""
+        ms = ctypes.c_double(0.0)
+        cnt = _c_i64(0)
+        self._check(self.lib.cipsi_ijkl_read_timing(self.handle, stage.encode(), ctypes.byref(ms),
+                                                    ctypes.byref(cnt)), "read_timing")
+        return ms.value, cnt.value
+
+    def tmono(self, h_packed, c):
+        cm, ldc = self._cmat(c)
+        n, m = c.shape
+        hp = np.ascontiguousarray(h_packed, dtype=np.float64)
+        out = np.empty(n_pairs(m))
+        self.lib.cipsi_ijkl_set_stream(self.handle, None)
+        self._check(self.lib.cipsi_ijkl_tmono(self.handle, n, m, _ptr(hp), _ptr(cm), ldc,
+                                              _ptr(out)), "tmono")
+        return out
+
+
+def workspace_bytes(n_ao, n_mo, precision="fp64", with_labels=False):
+    return load_library().cipsi_ijkl_workspace_bytes(n_ao, n_mo, _PREC[precision], int(with_labels))

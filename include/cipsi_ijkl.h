@@ -1,0 +1,128 @@
+/*
+ * cipsi_ijkl.h -- C ABI of the MI355X (gfx950) ijkl AO->MO integral transform.
+ *
+ * Drop-in for the `ijkl` step of the CIPSI chain (C1 case): same inputs
+ * (symmetry-packed AO integrals or the pshf label stream, MO coefficients) and the
+ * same packed (pq|rs) output order as the reference.  Plain C types only, callable
+ * from Fortran through ISO_C_BINDING (module cipsi_ijkl_mod,
+ * cipsi_amd/fortran/cipsi_ijkl_mod.f90).
+ *
+ * Layouts (0-based; pair0(a,b) = a(a+1)/2 + b for a >= b, P_A = N(N+1)/2, P_M = M(M+1)/2):
+ *   ao_packed  P_A(P_A+1)/2 doubles, ao_packed[pair0(pair0(i,j), pair0(k,l))] = (ij|kl)
+ *              for i>=j, k>=l, ij>=kl -- the block transd builds (transd.f:369-372).
+ *   c          column-major N_AO x N_MO, leading dimension ldc >= N_AO: the active MO
+ *              columns /om/ c (vector.f:38-41, after rotat2.f:50-60 frozen compaction).
+ *   h_half     P_A x P_M doubles, h_half[pq*P_M + pair0(l,k)] (tpqkld.f:91-102).
+ *   mo_packed  P_M(P_M+1)/2 doubles, mo_packed[pair0(kl, pair0(j,i))] = (ij|kl), i<=j,
+ *              ij<=kl, k<=l: the order tijkld streams to f50 (tijkld.f:159-186).
+ *
+ * Every entry point returns CIPSI_OK (0) or a negative status; none calls exit().
+ * Host-pointer entry points are synchronous and retain no pointer.  *_dev entry
+ * points take device pointers, are asynchronous on the context stream and are
+ * completed by cipsi_ijkl_synchronize().  A context is not thread-safe; use one
+ * context per host thread (the reference runs one process per geometry,
+ * Autocip_13/autocip13.f90:525-567).
+ */
+#ifndef CIPSI_IJKL_H
+#define CIPSI_IJKL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; the reference reports these conditions with `stop`
+   (ijkl.f:118-120, ijkl.f:304-316, itijkl.f:230-234, rotat2.f:347-380) */
+#define CIPSI_OK                  0
+#define CIPSI_ERR_ARG            -1   /* bad dimension, null pointer, ldc < n_ao, n_mo > n_ao */
+#define CIPSI_ERR_LABEL          -2   /* label stream: n_ao > 255 or a label outside the block */
+#define CIPSI_ERR_NOMEM          -3   /* device allocation failed */
+#define CIPSI_ERR_HIP            -4   /* other HIP runtime failure */
+#define CIPSI_ERR_NONFINITE      -5   /* NaN or Inf in an input */
+#define CIPSI_ERR_NOT_ORTHONORMAL -6  /* |C^T S C - 1| above tolerance */
+#define CIPSI_ERR_UNSUPPORTED    -7   /* n_ao > 256 or a precision / slice count not built */
+#define CIPSI_ERR_NODEVICE       -8   /* no usable gfx950 device */
+
+/* precision argument */
+#define CIPSI_PREC_FP64        0      /* FP64 MFMA (v_mfma_f64_16x16x4f64) */
+#define CIPSI_PREC_SPLIT_FP16  1      /* Ozaki FP16 slices on FP16 MFMA, FP32 accumulation */
+
+typedef struct cipsi_ijkl_ctx cipsi_ijkl_ctx;
+
+/* Context: owns device workspace, a HIP stream and tuned tile plans.
+   No counterpart in the reference (its state lives in COMMON blocks,
+   itijkl.f:12-22); needed so the library is re-entrant. */
+int cipsi_ijkl_create(cipsi_ijkl_ctx **ctx, int device);
+int cipsi_ijkl_destroy(cipsi_ijkl_ctx *ctx);
+const char *cipsi_ijkl_strerror(int status);
+const char *cipsi_ijkl_last_error(const cipsi_ijkl_ctx *ctx);
+/* Use an external hipStream_t (NULL restores the context's own stream). */
+int cipsi_ijkl_set_stream(cipsi_ijkl_ctx *ctx, void *hip_stream);
+int cipsi_ijkl_synchronize(cipsi_ijkl_ctx *ctx);
+/* Device bytes the context allocates for one transform of this size. */
+int64_t cipsi_ijkl_workspace_bytes(int n_ao, int n_mo, int precision, int with_labels);
+/* Release the cached workspace. */
+int cipsi_ijkl_release_workspace(cipsi_ijkl_ctx *ctx);
+
+/* Whole transform of a packed AO block: replaces `call itijkl(bijkl,bijkl,ydp)`
+   (ijkl.f:318) for C1 with YDP=T, i.e. tpqkld (itijkl.f:214-220) followed by tijkld
+   (itijkl.f:253-285), without the unit-40 scratch file.  err_est (may be NULL)
+   receives an a-priori bound on max |error| of the output. */
+int cipsi_ijkl_transform_packed(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo,
+                                const double *ao_packed, const double *c, int ldc,
+                                double *mo_packed, int precision, int n_slices,
+                                double *err_est);
+
+/* Same from the pshf/cval label stream (itijkl.f:57-68 + transd.f): xx and labels
+   hold n_entries values and 32-bit labels i<<24|j<<16|k<<8|l (iword_func.f:2-52,
+   1-based indices, values pre-divided as pshf/qout.f:73-77 does); the first zero
+   label ends the stream; a later duplicate overwrites an earlier one. */
+int cipsi_ijkl_transform_labels(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *xx,
+                                const int32_t *labels, int64_t n_entries, const double *c,
+                                int ldc, double *mo_packed, int precision, int n_slices,
+                                double *err_est);
+
+/* Device-pointer forms of the two entry points above. */
+int cipsi_ijkl_transform_packed_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo,
+                                    const double *d_ao_packed, const double *d_c, int ldc,
+                                    double *d_mo_packed, int precision, int n_slices);
+int cipsi_ijkl_transform_labels_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *d_xx,
+                                    const int32_t *d_labels, int64_t n_entries,
+                                    const double *d_c, int ldc, double *d_mo_packed,
+                                    int precision, int n_slices);
+
+/* transd.f (C1 route, transd.f:85-97,210-218,356-373): label stream -> packed AO block
+   on the device.  Returns CIPSI_ERR_LABEL (after synchronising) for a bad label. */
+int cipsi_ijkl_unpack_labels_dev(cipsi_ijkl_ctx *ctx, int n_ao, const double *d_xx,
+                                 const int32_t *d_labels, int64_t n_entries,
+                                 double *d_ao_packed);
+
+/* tpqkld.f:25-104: first half-transform, packed AO block -> h_half (P_A x P_M). */
+int cipsi_ijkl_half1_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *d_ao_packed,
+                         const double *d_c, int ldc, double *d_h_half, int precision,
+                         int n_slices);
+
+/* tijkld.f:28-88: second half-transform, h_half (P_A x P_M) -> packed MO stream. */
+int cipsi_ijkl_half2_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *d_h_half,
+                         const double *d_c, int ldc, double *d_mo_packed, int precision,
+                         int n_slices);
+
+/* tmono.f:39-54: h_MO = C^T h C, h packed N(N+1)/2 -> h_mo packed M(M+1)/2 (host). */
+int cipsi_ijkl_tmono(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *h_packed,
+                     const double *c, int ldc, double *h_mo_packed);
+
+/* Per-kernel GPU timing with HIP events recorded on the context stream around each
+   launch of the named stage ("expand", "half1", "transpose", "half2", "pad_c",
+   "unpack").  For benchmarking; no counterpart in the reference (whose timer calls
+   are commented out, itijkl.f:202-203,221-222). */
+int cipsi_ijkl_enable_timing(cipsi_ijkl_ctx *ctx, int enable);
+int cipsi_ijkl_read_timing(cipsi_ijkl_ctx *ctx, const char *stage, double *total_ms,
+                           int64_t *launches);
+int cipsi_ijkl_reset_timing(cipsi_ijkl_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CIPSI_IJKL_H */

@@ -1,0 +1,127 @@
+"""GPU parity of the HIP transform against the CPU oracle and the reference fixtures.
+
+Bar: FP64 path within 1e-12 absolute of the oracle (which is bit-exact with the
+reference program) on the same inputs; layout/ordering checks are exact.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cipsi_amd import formats, synth
+from tests.conftest import load_golden
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from cipsi_amd import ijkl
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = ijkl.Context(0)
+    yield c
+    c.close()
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def compacted_c(c_full, frozen):
+    n = c_full.shape[0]
+    if not frozen:
+        return np.asfortranarray(c_full)
+    return np.asfortranarray(np.concatenate([c_full[:, frozen:], c_full[:, n - frozen:]], axis=1))
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (5, 3), (12, 12), (17, 9), (33, 33), (40, 24)])
+def test_halves_match_tpqkld_tijkld(ctx, n, m):
+    print(f"[halves] n={n} m={m}", flush=True)
+    case = synth.make_case(n, m, max(2, 2 * n), 4.0, 100 + n + m)
+    V, c = case["ao_packed"], case["c"]
+    h_ref = oracle.tpqkld(V, c)
+    h = ctx.tpqkld(dev(V), dev(c))
+    torch.cuda.synchronize()
+    assert np.max(np.abs(h.cpu().numpy() - h_ref)) <= TOL
+    out_ref = oracle.tijkld(h_ref, c)
+    out = ctx.tijkld(dev(h_ref), dev(c))
+    torch.cuda.synchronize()
+    assert np.max(np.abs(out.cpu().numpy() - out_ref)) <= TOL
+
+
+@pytest.mark.parametrize("name", ["tiny_n12_frozen2", "sparse_n10_shuffled"])
+def test_labels_path_matches_reference_fixture(ctx, name):
+    g = load_golden(name)
+    n = int(g["n_ao"])
+    print(f"[labels] {name}", flush=True)
+    case = synth.make_case(n, int(g["n_mo"]), int(g["rank"]), float(g["decay"]), int(g["seed"]),
+                           frozen=int(g["frozen"]))
+    xx, ix = formats.labels_from_packed(case["ao_packed"], n, cutoff=float(g["cutoff"]))
+    if bool(g["shuffle"]):
+        perm = np.random.default_rng(int(g["seed"]) + 3).permutation(xx.size)
+        xx, ix = xx[perm], ix[perm]
+    xx = np.append(xx, 0.0)
+    ix = np.append(ix, 0).astype(np.int32)
+    cc = compacted_c(g["c_full"], int(g["frozen"]))
+    out = ctx.itijkl(xx, ix, cc)                       # host-pointer entry point
+    assert np.max(np.abs(out - g["mo"])) <= TOL
+    out_d = ctx.itijkl(dev(xx), dev(ix), dev(cc))       # device entry point
+    torch.cuda.synchronize()
+    assert np.max(np.abs(out_d.cpu().numpy() - g["mo"])) <= TOL
+    # the unpacked block equals the oracle's transd exactly
+    blk = ctx.transd(dev(xx), dev(ix), n)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(blk.cpu().numpy(), oracle.transd(n, xx, ix))
+    np.testing.assert_allclose(ctx.tmono(g["h_packed"], cc), g["hmo"], rtol=0, atol=TOL)
+
+
+def test_cfg0_n40_against_reference_and_oracle(ctx):
+    g = load_golden("cfg0_n40")
+    print("[cfg0] n=40", flush=True)
+    case = synth.make_case(40, 40, 120, 8.0, 2024)
+    c = compacted_c(g["c_full"], 0)
+    out = ctx.transform(case["ao_packed"], c)
+    ref = oracle.ijkl(case["ao_packed"], c)
+    assert np.max(np.abs(out - ref)) <= TOL
+    assert np.max(np.abs(out[g["sample_idx"]] - g["sample_val"])) <= TOL
+    assert abs(np.sum(out) - g["full_sum"]) <= 1e-8
+
+
+def test_cfg1_n120_frozen10_against_reference(ctx):
+    g = load_golden("cfg1_n120_frozen10")
+    print("[cfg1] n=120 m=110", flush=True)
+    case = synth.make_case(120, 110, 360, 15.0, 2025, frozen=10)
+    out = ctx.transform(dev(case["ao_packed"]), dev(case["c"]))
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    assert np.max(np.abs(o[g["sample_idx"]] - g["sample_val"])) <= TOL
+    assert abs(np.sum(o) - g["active_sum"]) <= 1e-6
+    assert abs(np.sum(o * o) - g["active_sumsq"]) <= 1e-6 * g["active_sumsq"]
+    # whole stream against the independent closed form (different rounding path)
+    cf = synth.mo_reference_closed_form(case["B"], case["c"])
+    assert np.max(np.abs(o - cf)) <= 1e-11
+
+
+def test_errors_reported_not_fatal(ctx):
+    from cipsi_amd import ijkl
+    c = np.eye(4)[:, :3]
+    V = synth.make_case(4, 3, 4, 2.0, 1)["ao_packed"]
+    V3 = synth.make_case(3, 3, 4, 2.0, 1)["ao_packed"]
+    with pytest.raises(ijkl.IjklError) as e:
+        ctx.transform(V3, np.eye(3, 4))            # n_mo > n_ao
+    assert e.value.status == -1
+    with pytest.raises(ijkl.IjklError) as e:
+        bad = V.copy()
+        bad[3] = np.nan
+        ctx.transform(bad, c)
+    assert e.value.status == -5
+    with pytest.raises(ijkl.IjklError) as e:
+        ctx.itijkl(np.array([1.0, 0.0]), np.array([formats.label_encode(5, 1, 1, 1), 0],
+                                                   dtype=np.int32), c)
+    assert e.value.status == -2
+    # the context is still usable afterwards
+    out = ctx.transform(V, c)
+    assert np.max(np.abs(out - oracle.ijkl(V, c))) <= TOL
