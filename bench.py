@@ -53,7 +53,7 @@ def make_inputs(n, m, rank, decay, seed, torch):
     return B, ao, torch.as_tensor(np.ascontiguousarray(c), device="cuda"), c
 
 
-def cpu_baseline(ao_host, c, h_gpu, out_gpu, n, m, nthreads=1, n_pq=240, n_kl=320):
+def cpu_baseline(ao_host, c, h_gpu, out_gpu, n, m, nthreads=1, n_pq=1200, n_kl=1600):
     """Time the oracle on sampled AO pairs (half 1) and MO pairs (half 2) of the same
     workload and extrapolate to a whole transform.  Cost per AO pair is constant;
     MO pairs are taken with a fixed stride so the sample is unbiased over ilk."""

@@ -230,7 +230,13 @@ int cipsi_ijkl_destroy(cipsi_ijkl_ctx* ctx) {
 
 int cipsi_ijkl_set_stream(cipsi_ijkl_ctx* ctx, void* s) {
     if (!ctx) return CIPSI_ERR_ARG;
-    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    ctx->stream = (hipStream_t)s;
+    return CIPSI_OK;
+}
+
+int cipsi_ijkl_reset_stream(cipsi_ijkl_ctx* ctx) {
+    if (!ctx) return CIPSI_ERR_ARG;
+    ctx->stream = ctx->own_stream;
     return CIPSI_OK;
 }
 

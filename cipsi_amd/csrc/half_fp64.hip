@@ -7,7 +7,8 @@
 // Z_b panels (16 rows of the symmetric matrix) and the matching C rows are staged
 // through LDS with register prefetch of the next panel; T stays in LDS for the
 // second quarter.  Both quarters run on v_mfma_f64_16x16x4f64 (A: lane l holds
-// A[l%16][l/16]; B: lane l holds B[l/16][l%16]; D: lane l holds rows 4(l/16)+i, col l%16).
+// A[l%16][l/16]; B: lane l holds B[l/16][l%16]; D: element q of lane l is
+// D[l/16 + 4q][l%16]).
 #include "common.h"
 
 namespace cipsi {
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(NW * 64) void k_half_fp64(HalfArgs A) {
             for (int j = 0; j < K::NCB; ++j)
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    ts[(rb * 16 + lg * 4 + q) * K::LDT + j * 16 + lm] = acc[i][j][q];
+                    ts[(rb * 16 + lg + 4 * q) * K::LDT + j * 16 + lm] = acc[i][j][q];
         }
     }
     __syncthreads();
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(NW * 64) void k_half_fp64(HalfArgs A) {
             if (c >= c1) continue;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int a = ab * 16 + lg * 4 + q;
+                const int a = ab * 16 + lg + 4 * q;
                 const int64_t p = pair0(c, a);
                 if (a <= c && p < limit) dst[p] = o[j][q];
             }

@@ -36,7 +36,7 @@ STATUS = {0: "CIPSI_OK", -1: "CIPSI_ERR_ARG", -2: "CIPSI_ERR_LABEL", -3: "CIPSI_
 # every symbol include/cipsi_ijkl.h declares
 EXPORTS = [
     "cipsi_ijkl_create", "cipsi_ijkl_destroy", "cipsi_ijkl_strerror", "cipsi_ijkl_last_error",
-    "cipsi_ijkl_set_stream", "cipsi_ijkl_synchronize", "cipsi_ijkl_workspace_bytes",
+    "cipsi_ijkl_set_stream", "cipsi_ijkl_reset_stream", "cipsi_ijkl_synchronize", "cipsi_ijkl_workspace_bytes",
     "cipsi_ijkl_release_workspace", "cipsi_ijkl_transform_packed", "cipsi_ijkl_transform_labels",
     "cipsi_ijkl_transform_packed_dev", "cipsi_ijkl_transform_labels_dev",
     "cipsi_ijkl_unpack_labels_dev", "cipsi_ijkl_half1_dev", "cipsi_ijkl_half2_dev",
@@ -74,6 +74,7 @@ def load_library(path: str = LIB_PATH):
     L.cipsi_ijkl_last_error.restype = ctypes.c_char_p
     L.cipsi_ijkl_last_error.argtypes = [_vp]
     L.cipsi_ijkl_set_stream.argtypes = [_vp, _vp]
+    L.cipsi_ijkl_reset_stream.argtypes = [_vp]
     L.cipsi_ijkl_synchronize.argtypes = [_vp]
     L.cipsi_ijkl_workspace_bytes.restype = _c_i64
     L.cipsi_ijkl_workspace_bytes.argtypes = [_c_int, _c_int, _c_int, _c_int]
@@ -141,6 +142,7 @@ class Context:
             raise IjklError(rc, what, detail or self.lib.cipsi_ijkl_strerror(rc).decode())
 
     def _use_torch_stream(self):
+        """Order library work with torch's current stream (handle 0 is the null stream)."""
         import torch
         self.lib.cipsi_ijkl_set_stream(self.handle, _vp(torch.cuda.current_stream().cuda_stream))
 
@@ -184,7 +186,7 @@ class Context:
             raise ValueError("ao_packed has the wrong length")
         out = np.empty(nout)
         err = ctypes.c_double(0.0)
-        self.lib.cipsi_ijkl_set_stream(self.handle, None)
+        self.lib.cipsi_ijkl_reset_stream(self.handle)
         self._check(self.lib.cipsi_ijkl_transform_packed(
             self.handle, n, m, _ptr(ao), _ptr(cm), ldc, _ptr(out), prec, n_slices,
             ctypes.byref(err)), "transform_packed")
@@ -208,7 +210,7 @@ class Context:
         ix = np.ascontiguousarray(ix, dtype=np.int32)
         out = np.empty(nout)
         err = ctypes.c_double(0.0)
-        self.lib.cipsi_ijkl_set_stream(self.handle, None)
+        self.lib.cipsi_ijkl_reset_stream(self.handle)
         self._check(self.lib.cipsi_ijkl_transform_labels(
             self.handle, n, m, _ptr(xx), _ptr(ix), xx.size, _ptr(cm), ldc, _ptr(out), prec,
             n_slices, ctypes.byref(err)), "transform_labels")
@@ -268,7 +270,7 @@ class Context:
         n, m = c.shape
         hp = np.ascontiguousarray(h_packed, dtype=np.float64)
         out = np.empty(n_pairs(m))
-        self.lib.cipsi_ijkl_set_stream(self.handle, None)
+        self.lib.cipsi_ijkl_reset_stream(self.handle)
         self._check(self.lib.cipsi_ijkl_tmono(self.handle, n, m, _ptr(hp), _ptr(cm), ldc,
                                               _ptr(out)), "tmono")
         return out

@@ -57,8 +57,11 @@ int cipsi_ijkl_create(cipsi_ijkl_ctx **ctx, int device);
 int cipsi_ijkl_destroy(cipsi_ijkl_ctx *ctx);
 const char *cipsi_ijkl_strerror(int status);
 const char *cipsi_ijkl_last_error(const cipsi_ijkl_ctx *ctx);
-/* Use an external hipStream_t (NULL restores the context's own stream). */
+/* Run subsequent work on an external hipStream_t, taken as given: NULL is the HIP
+   null stream (ordered with every blocking stream of the device). */
 int cipsi_ijkl_set_stream(cipsi_ijkl_ctx *ctx, void *hip_stream);
+/* Return to the context's own (non-blocking) stream. */
+int cipsi_ijkl_reset_stream(cipsi_ijkl_ctx *ctx);
 int cipsi_ijkl_synchronize(cipsi_ijkl_ctx *ctx);
 /* Device bytes the context allocates for one transform of this size. */
 int64_t cipsi_ijkl_workspace_bytes(int n_ao, int n_mo, int precision, int with_labels);

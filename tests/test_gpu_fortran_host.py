@@ -1,0 +1,58 @@
+"""The Fortran host program (cipsi_amd/fortran/ijkl_gpu) as a drop-in for the
+reference ijkl executable: same stdin namelists and input files, outputs compared
+with the reference's own f50 / f25 / ijom (golden fixtures)."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from cipsi_amd import formats, synth
+from tests.conftest import ROOT, load_golden
+
+pytestmark = pytest.mark.gpu
+HOST = os.path.join(ROOT, "cipsi_amd", "fortran", "ijkl_gpu")
+
+
+@pytest.mark.parametrize("name", ["tiny_n12_frozen2", "sparse_n10_shuffled", "cfg0_n40"])
+def test_fortran_host_matches_reference_files(name):
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    if not os.path.exists(HOST):
+        pytest.fail("ijkl_gpu is not built")
+    print(f"[fortran host] {name}", flush=True)
+    g = load_golden(name)
+    n, frozen = int(g["n_ao"]), int(g["frozen"])
+    case = synth.make_case(n, int(g["n_mo"]), int(g["rank"]), float(g["decay"]), int(g["seed"]),
+                           frozen=frozen)
+    xx, ix = formats.labels_from_packed(case["ao_packed"], n, cutoff=float(g["cutoff"]))
+    if bool(g["shuffle"]):
+        perm = np.random.default_rng(int(g["seed"]) + 3).permutation(xx.size)
+        xx, ix = xx[perm], ix[perm]
+    with tempfile.TemporaryDirectory() as wd:
+        formats.write_info(os.path.join(wd, "info"), n, g["c_full"], g["h_packed"], enuc=1.25,
+                           n_elec=2)
+        nrec = formats.write_pqrs(os.path.join(wd, "pqrs"), xx, ix)
+        stdin = formats.ijfil_namelist() + formats.ijklin_namelist(nrec, nvecf=n, frozen=frozen)
+        res = subprocess.run([HOST], input=stdin.encode(), cwd=wd, capture_output=True, timeout=120)
+        log = res.stdout.decode(errors="replace")
+        assert res.returncode == 0 and "fin ijkl" in log, log[-2000:] + res.stderr.decode()
+        f50 = formats.read_f50(os.path.join(wd, "f50"), n)
+        with open(os.path.join(wd, "f25"), "rb") as f:
+            f25 = f.read()
+        ijom = formats.read_records(os.path.join(wd, "ijom"))
+    assert f50["n_records"] == int(g["n_records"])
+    assert f25 == g["f25_bytes"].tobytes()
+    np.testing.assert_array_equal(np.frombuffer(ijom[0], "<f8"), g["ijom_c"])
+    np.testing.assert_array_equal(np.frombuffer(ijom[2], "<f8"), g["ijom_s"])
+    assert np.max(np.abs(f50["hmo"] - g["hmo"])) <= 1e-12
+    if "mo" in g:
+        assert np.max(np.abs(f50["mo"] - g["mo"])) <= 1e-12
+        assert np.max(np.abs(f50["tail"] - g["tail"])) <= 1e-12
+    else:
+        assert np.max(np.abs(f50["mo"][g["sample_idx"]] - g["sample_val"])) <= 1e-12
+        nr = int(g["n_mo_records"])
+        assert np.max(np.abs(f50["mo"][(nr - 1) * formats.LSIZE:] - g["last_record"])) <= 1e-12
+        assert np.max(np.abs(f50["tail"] - g["tail"])) <= 1e-12

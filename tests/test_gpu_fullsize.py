@@ -1,0 +1,95 @@
+"""Full-size checks (N_AO = N_MO = 240, rank 720) and stream ordering with torch.
+
+At this size the serial oracle would take minutes, so parity is pinned by
+* the whole packed MO stream against the closed form (pq|rs) = sum_P Bt^P_pq Bt^P_rs,
+  Bt^P = C^T B^P C, computed on the GPU with torch (independent rounding path);
+* sampled AO-pair rows of H and sampled MO-pair rows of the output against the
+  oracle's tpqkld / tijkld on the same inputs (the same restatement that is
+  bit-exact with the reference program).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cipsi_amd import synth
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from cipsi_amd import ijkl
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = ijkl.Context(0)
+    yield c
+    c.close()
+
+
+def closed_form_packed(B, c):
+    """Packed MO stream from the ERI factors, on the GPU."""
+    n, m = c.shape
+    a, b = np.tril_indices(n)
+    am, bm = np.tril_indices(m)
+    Bd = torch.as_tensor(B, device="cuda")
+    cd = torch.as_tensor(np.ascontiguousarray(c), device="cuda")
+    ia = torch.as_tensor(a, device="cuda")
+    ib = torch.as_tensor(b, device="cuda")
+    out_f = torch.empty((B.shape[0], am.size), dtype=torch.float64, device="cuda")
+    for p0 in range(0, B.shape[0], 64):
+        p1 = min(B.shape[0], p0 + 64)
+        full = torch.zeros((p1 - p0, n, n), dtype=torch.float64, device="cuda")
+        full[:, ia, ib] = Bd[p0:p1]
+        full[:, ib, ia] = Bd[p0:p1]
+        t = torch.matmul(torch.matmul(cd.t().contiguous(), full), cd)
+        out_f[p0:p1] = t[:, torch.as_tensor(am, device="cuda"), torch.as_tensor(bm, device="cuda")]
+    return synth.eri_packed_torch(out_f.cpu().numpy(), "cuda")
+
+
+def test_stream_ordering_without_device_sync(ctx):
+    """Results read through torch's own stream (.cpu()) without a device-wide sync."""
+    n = 160
+    print(f"[stream] n={n}", flush=True)
+    B = synth.eri_factors(n, 2 * n, 10.0, 31)
+    V = synth.eri_packed_torch(B, "cuda")
+    c = synth.orthonormal(n, 32)
+    h = ctx.tpqkld(V, torch.as_tensor(c, device="cuda")).cpu().numpy()
+    a, b = np.tril_indices(n)
+    for pq in (0, 1234, n * (n + 1) // 2 - 1):
+        row = B[:, pq] @ B
+        Z = np.zeros((n, n))
+        Z[a, b] = row
+        Z[b, a] = row
+        O = c.T @ Z @ c
+        assert np.max(np.abs(h[pq] - O[a, b])) <= 1e-11
+
+
+def test_cfg2_n240_fp64_full_stream(ctx):
+    n = m = 240
+    print("[cfg2] n=240 fp64 full stream", flush=True)
+    cfg = synth.CONFIGS[2]
+    B = synth.eri_factors(n, cfg["rank"], cfg["decay"], cfg["seed"])
+    V = synth.eri_packed_torch(B, "cuda")
+    c = synth.orthonormal(n, cfg["seed"] + 1)
+    cd = torch.as_tensor(c, device="cuda")
+    out = ctx.transform(V, cd)
+    ref = closed_form_packed(B, c)
+    diff = float(torch.max(torch.abs(out - ref)))
+    print(f"  max|gpu - closed form| = {diff:.3e}, max|out| = {float(torch.max(torch.abs(ref))):.3f}")
+    assert diff <= 1e-11
+    del ref
+    # sampled rows of both halves against the oracle restatement
+    h = ctx.tpqkld(V, cd)
+    h_host = h.cpu().numpy()
+    del h
+    pa = n * (n + 1) // 2
+    pq_list = np.array([0, 1, 4000, 17777, pa - 1], dtype=np.int64)
+    V_host = V.cpu().numpy()
+    h_ref = oracle.tpqkld_rows(V_host, c, pq_list)
+    assert np.max(np.abs(h_ref[pq_list] - h_host[pq_list])) <= 1e-12
+    kl_list = np.array([0, 2, 999, 20000, pa - 1], dtype=np.int64)
+    o_ref = oracle.tijkld_rows(h_host, c, kl_list)
+    o = out.cpu().numpy()
+    idx = np.concatenate([np.arange(k * (k + 1) // 2, k * (k + 1) // 2 + k + 1) for k in kl_list])
+    assert np.max(np.abs(o_ref[idx] - o[idx])) <= 1e-12
