@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--n-mo", type=int, default=240)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -101,6 +103,15 @@ def main():
     print(f"[bench] generating N_AO={n} N_MO={m} rank={rank}", file=sys.stderr, flush=True)
     B, ao, c_dev, c_host = make_inputs(n, m, rank, decay, seed, torch)
     torch.cuda.synchronize()
+
+    if args.calib_copy_mib:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE scale the counters
+        src = torch.ones(args.calib_copy_mib * 2**17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ctx = ijkl.Context(0)
     prec = args.precision

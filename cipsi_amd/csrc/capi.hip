@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -88,9 +89,8 @@ int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int6
     return CIPSI_OK;
 }
 
-int ctx_half_lt(const cipsi_ijkl_ctx* ctx, int half, int n, int m) {
-    (void)n; (void)m;
-    return half == 1 ? ctx->lt_half1 : ctx->lt_half2;
+int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half) {
+    return half == 1 ? ctx->var_half1 : ctx->var_half2;
 }
 
 // --------------------------------------------------------------------------- pipeline pieces
@@ -119,7 +119,8 @@ int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, con
     a.mp = round16(m);
     a.half = half;
     a.b_begin = 0;
-    const int lt = ctx_half_lt(ctx, half, n, m);
+    const int var = ctx_half_variant(ctx, half);
+    const int lt = variant_columns(var);
     if (half == 1) {
         a.nb = pa;
         a.dst_ld = pm;
@@ -137,7 +138,7 @@ int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, con
     }
     StageTimer tm(ctx, half == 1 ? "half1" : "half2");
     if (precision == CIPSI_PREC_FP64) {
-        HIP_TRY(ctx, launch_half_fp64(a, lt, ctx->stream));
+        HIP_TRY(ctx, launch_half_fp64(a, var, ctx->stream));
         return CIPSI_OK;
     }
     if (precision == CIPSI_PREC_SPLIT_FP16) {
@@ -195,6 +196,14 @@ int cipsi_ijkl_create(cipsi_ijkl_ctx** out, int device) {
         return CIPSI_ERR_HIP;
     }
     ctx->stream = ctx->own_stream;
+    if (const char* v = std::getenv("CIPSI_IJKL_VARIANTS")) {     // "h1,h2" kernel variants
+        int v1 = -1, v2 = -1;
+        if (std::sscanf(v, "%d,%d", &v1, &v2) == 2 && v1 >= 0 && v1 < kNumHalfVariants && v2 >= 0 &&
+            v2 < kNumHalfVariants) {
+            ctx->var_half1 = v1;
+            ctx->var_half2 = v2;
+        }
+    }
     *out = ctx;
     return CIPSI_OK;
 }
@@ -479,8 +488,8 @@ int cipsi_ijkl_tmono(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* h_pa
     a.half = 1;
     a.nb = 1;
     a.dst_ld = pm;
-    a.ntile_total = (n_mo + 31) / 32;
-    HIP_TRY(ctx, launch_half_fp64(a, 32, ctx->stream));
+    a.ntile_total = (n_mo + variant_columns(0) - 1) / variant_columns(0);
+    HIP_TRY(ctx, launch_half_fp64(a, 0, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(h_mo, ctx->io_out.ptr, sizeof(double) * pm, hipMemcpyDeviceToHost,
                                 ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

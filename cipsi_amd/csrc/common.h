@@ -54,6 +54,17 @@ struct HalfArgs {
     const int64_t* tile_start;  // half 2: prefix table over l (m + 1 entries), device
 };
 
+// A tile variant of the half-transform kernel: waves per work-group (16 output
+// columns each), Q1 panel depth, and the work-groups-per-CU occupancy hint.
+struct HalfVariant {
+    int waves;
+    int kc;
+    int minb;
+};
+extern const HalfVariant kHalfVariants[];
+extern const int kNumHalfVariants;
+inline int variant_columns(int v) { return 16 * kHalfVariants[v].waves; }
+
 // Device launchers (defined in the .hip files).
 hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t n_entries,
                                 int n_ao, double* ao_packed, uint32_t* owner, int64_t* first_zero,
@@ -63,7 +74,7 @@ hipError_t launch_transpose(const double* src, int64_t rows, int64_t cols, doubl
                             hipStream_t st);
 hipError_t launch_pad_c(const double* c, int ldc, int n, int m, int np, int mp, double* cr,
                         hipStream_t st);
-hipError_t launch_half_fp64(const HalfArgs& a, int lt, hipStream_t st);
+hipError_t launch_half_fp64(const HalfArgs& a, int variant, hipStream_t st);
 hipError_t launch_zero(double* p, int64_t n, hipStream_t st);
 
 }  // namespace cipsi

@@ -31,9 +31,9 @@ struct cipsi_ijkl_ctx {
     DevBuf slices;   // split-FP16 operand slices
     int ts_m = -1, ts_lt = -1;
     int64_t ts_total = 0;
-    // tile plans (columns per work-group) for the two halves
-    int lt_half1 = 32;
-    int lt_half2 = 32;
+    // tile plans (kernel variant index) for the two halves
+    int var_half1 = 3;
+    int var_half2 = 3;
     // per-stage event timing
     bool timing = false;
     struct Pending { std::string stage; hipEvent_t a, b; };

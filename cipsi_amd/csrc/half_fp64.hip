@@ -1,29 +1,38 @@
 // FP64 MFMA half-transform ("sandwich") kernel for gfx950.
 //
 // One work-group = one symmetric matrix Z_b (an AO pair for half 1, an MO pair for
-// half 2) times one tile of LT output columns:
-//   Q1/Q3:  T = Z_b * C[:, c0:c0+LT]                (tpqkld.f:82-90 / tijkld.f:53-60)
-//   Q2/Q4:  O(a, c) = sum_r C(r, a) T(r, c), a <= c  (tpqkld.f:91-97 / tijkld.f:61-76)
-// Z_b panels (16 rows of the symmetric matrix) and the matching C rows are staged
-// through LDS with register prefetch of the next panel; T stays in LDS for the
-// second quarter.  Both quarters run on v_mfma_f64_16x16x4f64 (A: lane l holds
-// A[l%16][l/16]; B: lane l holds B[l/16][l%16]; D: element q of lane l is
-// D[l/16 + 4q][l%16]).
+// half 2) times one tile of NW*16 output columns; wave w owns the 16 columns
+// c_w = c0 + 16w .. c_w + 15 and every row block:
+//   Q1/Q3:  T(r, c) = sum_s Z_b(r, s) C(s, c)          (tpqkld.f:82-90 / tijkld.f:53-60)
+//   Q2/Q4:  O(a, c) = sum_r C(r, a) T(r, c), a <= c    (tpqkld.f:91-97 / tijkld.f:61-76)
+//
+// v_mfma_f64_16x16x4f64 lane layouts (l = lane, lm = l%16, lg = l/16):
+//   A[m][k] = lane (m, k) -> A[lm][lg];  B[k][n] -> B[lg][lm];
+//   D element q of lane l  = D[lg + 4q][lm].
+// The D layout of a T row block is therefore exactly the operand layout of the four
+// K-steps r = 16rb + 4q + {0..3}: T never leaves the registers.  Q2 is computed as
+// O^T(c, a) = sum_r T(r, c) C(r, a) with T as the A operand, so a lane's results are
+// 16 consecutive a of one c -- 128-byte coalesced stores into the packed output.
+//
+// Z_b panels (KC columns s of the symmetric matrix, all rows) and the matching rows
+// of C are staged through LDS with register prefetch of the next panel; in Q2 the
+// 16-column blocks of C are staged the same way and shared by the waves.
 #include "common.h"
 
 namespace cipsi {
 
-constexpr int KC = 16;            // panel depth (rows of Z per LDS stage)
-
-template <int LT, int NW>
+template <int NW, int NRB, int KC>
 struct HalfCfg {
     static constexpr int NT = NW * 64;
-    static constexpr int NCB = LT / 16;
-    static constexpr int RBMAX = (16 + NW - 1) / NW;      // np <= 256 -> <= 16 row blocks
+    static constexpr int LT = NW * 16;                       // columns per work-group
     static constexpr int LDC = LT + 2;
-    static constexpr int LDT = LT + 2;
-    static constexpr int EPT_Z = (256 * KC + NT - 1) / NT;  // Z panel elements per thread
-    static constexpr int EPT_C = (KC * LT + NT - 1) / NT;   // C panel elements per thread
+    static constexpr int EP = 16 * KC;                       // panel elements per 16-row pass
+    static constexpr int PPS = NT >= EP ? NT / EP : 1;       // passes per sweep of the threads
+    static constexpr int SUB = NT >= EP ? 1 : EP / NT;       // sweeps per pass
+    static constexpr int NU = ((NRB + PPS - 1) / PPS) * SUB; // loads per thread per panel
+    static constexpr int EPT_C = (KC * LT + NT - 1) / NT;    // C panel elements per thread
+    static constexpr int LDQ = 18;                           // Q2 C-block row stride
+    static constexpr int EPT_Q = (NRB * 16 * 16 + NT - 1) / NT;
 };
 
 __device__ inline void work_item(const HalfArgs& A, int lt, int64_t item, int64_t& b, int& tile,
@@ -52,9 +61,9 @@ __device__ inline void work_item(const HalfArgs& A, int lt, int64_t item, int64_
     }
 }
 
-template <int LT, int NW>
-__global__ __launch_bounds__(NW * 64) void k_half_fp64(HalfArgs A) {
-    using K = HalfCfg<LT, NW>;
+template <int NW, int NRB, int KC, int MINB>
+__global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
+    using K = HalfCfg<NW, NRB, KC>;
     extern __shared__ double smem[];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
@@ -69,191 +78,215 @@ __global__ __launch_bounds__(NW * 64) void k_half_fp64(HalfArgs A) {
     if (item >= A.ntile_total) return;
     int64_t b, limit, dst_off;
     int tile, ncols;
-    work_item(A, LT, item, b, tile, ncols, limit, dst_off);
-    const int c0 = tile * LT;
-    const int c1 = min(c0 + LT, ncols);
+    work_item(A, K::LT, item, b, tile, ncols, limit, dst_off);
+    const int c0 = tile * K::LT;
+    const int c1 = min(c0 + K::LT, ncols);
+    const int cw = c0 + 16 * w;                    // first column of this wave
+    const bool wave_active = cw < c1;
     const double* __restrict__ zsrc = A.src + b * A.src_ld;
     const double* __restrict__ cr = A.cr;
     const int ldm = A.ldm;
-
-    double* zs_buf[2] = {smem, smem + (KC * LDZ + KC * K::LDC)};
-
-    // ---- panel prefetch into registers: Z(r, s0+kk) for r < np, kk < KC; C(s0+kk, c0+c)
-    double zst[K::EPT_Z];
-    int zoff[K::EPT_Z];
-    double cst[K::EPT_C];
-    auto fetch = [&](int s0) {
-        const int cntA = (np - s0) * KC;      // rows r >= s0: contiguous along s
-#pragma unroll
-        for (int t = 0; t < K::EPT_Z; ++t) {
-            const int e = tid + t * K::NT;
-            double v = 0.0;
-            int off = -1;
-            if (e < np * KC) {
-                int r, kk;
-                if (e < cntA) { r = s0 + e / KC; kk = e % KC; }
-                else { const int e2 = e - cntA; kk = e2 / s0; r = e2 - kk * s0; }
-                const int s = s0 + kk;
-                if (r < n && s < n) v = zsrc[r >= s ? pair0(r, s) : pair0(s, r)];
-                off = kk * LDZ + r;
-            }
-            zst[t] = v;
-            zoff[t] = off;
-        }
-#pragma unroll
-        for (int t = 0; t < K::EPT_C; ++t) {
-            const int e = tid + t * K::NT;
-            double v = 0.0;
-            if (e < KC * LT) {
-                const int kk = e / LT, c = e % LT;
-                if (c0 + c < A.mp) v = cr[(int64_t)(s0 + kk) * ldm + c0 + c];
-            }
-            cst[t] = v;
-        }
-    };
-    auto commit = [&](double* buf) {
-        double* zs = buf;
-        double* cs = buf + KC * LDZ;
-#pragma unroll
-        for (int t = 0; t < K::EPT_Z; ++t)
-            if (zoff[t] >= 0) zs[zoff[t]] = zst[t];
-#pragma unroll
-        for (int t = 0; t < K::EPT_C; ++t) {
-            const int e = tid + t * K::NT;
-            if (e < KC * LT) cs[(e / LT) * K::LDC + (e % LT)] = cst[t];
-        }
-    };
-
-    // ---- Q1: T = Z * C[:, c0:c0+LT]
-    f64x4 acc[K::RBMAX][K::NCB];
-#pragma unroll
-    for (int i = 0; i < K::RBMAX; ++i)
-#pragma unroll
-        for (int j = 0; j < K::NCB; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-
     const int nrb = np >> 4;
+
+    const int panel = KC * LDZ + KC * K::LDC;
+    double* buf[2] = {smem, smem + panel};
+
+    // ---------------------------------------------------------------- Q1 staging
+    // Panel = Z(r, s0 + kk) for r < np, kk < KC, taken in 16-row passes.  Below the
+    // diagonal block (r >= s) a row's KC entries are contiguous in the packed row
+    // (kk fastest across lanes); above it (r < s0) the entries of one s are contiguous
+    // in r (r fastest across lanes).  Branch-free: invalid elements load index 0.
+    double zst[K::NU];
+    double cst[K::EPT_C];
+    // load slot u of this thread -> (pass p, element e within the pass)
+    auto zslot = [&](int u, int& p, int& e) {
+        const int sweep = u / K::SUB, sub = u % K::SUB;
+        const int t = tid + sub * K::NT;
+        p = sweep * K::PPS + t / K::EP;
+        e = t % K::EP;
+    };
+    auto zmap = [&](int p, int e, int s0, int& r, int& kk) {
+        if (16 * p + 15 < s0) { r = 16 * p + (e & 15); kk = e >> 4; }
+        else { r = 16 * p + e / KC; kk = e % KC; }
+    };
+    auto fetch = [&](int s0) {
+#pragma unroll
+        for (int u = 0; u < K::NU; ++u) {
+            int p, e, r, kk;
+            zslot(u, p, e);
+            zmap(p, e, s0, r, kk);
+            const int s = s0 + kk;
+            const bool ok = (p < nrb) && (r < n) && (s < n);
+            const int hi = r >= s ? r : s, lo = r >= s ? s : r;
+            const int idx = ok ? ((hi * (hi + 1)) >> 1) + lo : 0;
+            const double v = zsrc[idx];
+            zst[u] = ok ? v : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < K::EPT_C; ++t) {
+            const int e = tid + t * K::NT;
+            const int kk = e / K::LT, c = e % K::LT;
+            const bool ok = (e < KC * K::LT) && (c0 + c < A.mp);
+            const double v = cr[ok ? (int64_t)(s0 + kk) * ldm + c0 + c : 0];
+            cst[t] = ok ? v : 0.0;
+        }
+    };
+    auto commit = [&](int s0, double* bp) {
+#pragma unroll
+        for (int u = 0; u < K::NU; ++u) {
+            int p, e, r, kk;
+            zslot(u, p, e);
+            zmap(p, e, s0, r, kk);
+            if (p < nrb) bp[kk * LDZ + r] = zst[u];
+        }
+        double* cs = bp + KC * LDZ;
+#pragma unroll
+        for (int t = 0; t < K::EPT_C; ++t) {
+            const int e = tid + t * K::NT;
+            if (e < KC * K::LT) cs[(e / K::LT) * K::LDC + (e % K::LT)] = cst[t];
+        }
+    };
+
+    // ---------------------------------------------------------------- Q1: T in registers
+    f64x4 acc[NRB];
+#pragma unroll
+    for (int i = 0; i < NRB; ++i) acc[i] = f64x4{0.0, 0.0, 0.0, 0.0};
+
     const int nchunk = np / KC;
     fetch(0);
-    commit(zs_buf[0]);
+    commit(0, buf[0]);
     __syncthreads();
     for (int ch = 0; ch < nchunk; ++ch) {
         const bool more = (ch + 1) < nchunk;
         if (more) fetch((ch + 1) * KC);
-        const double* zs = zs_buf[ch & 1];
-        const double* cs = zs + KC * LDZ;
+        if (wave_active) {
+            const double* zs = buf[ch & 1];
+            const double* cs = zs + KC * LDZ;
 #pragma unroll
-        for (int k4 = 0; k4 < KC; k4 += 4) {
-            double bf[K::NCB];
+            for (int k4 = 0; k4 < KC; k4 += 4) {
+                const double bf = cs[(k4 + lg) * K::LDC + 16 * w + lm];
 #pragma unroll
-            for (int j = 0; j < K::NCB; ++j) bf[j] = cs[(k4 + lg) * K::LDC + j * 16 + lm];
-#pragma unroll
-            for (int i = 0; i < K::RBMAX; ++i) {
-                const int rb = w + i * NW;
-                if (rb < nrb) {
-                    const double af = zs[(k4 + lg) * LDZ + rb * 16 + lm];
-#pragma unroll
-                    for (int j = 0; j < K::NCB; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf[j], acc[i][j], 0, 0, 0);
-                }
-            }
-        }
-        if (more) commit(zs_buf[(ch + 1) & 1]);
-        __syncthreads();
-    }
-
-    // ---- T -> LDS (aliases the panel buffers; all reads finished at the barrier above)
-    double* ts = smem;
-#pragma unroll
-    for (int i = 0; i < K::RBMAX; ++i) {
-        const int rb = w + i * NW;
-        if (rb < nrb) {
-#pragma unroll
-            for (int j = 0; j < K::NCB; ++j)
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    ts[(rb * 16 + lg + 4 * q) * K::LDT + j * 16 + lm] = acc[i][j][q];
-        }
-    }
-    __syncthreads();
-
-    // ---- Q2: O(a, c) = sum_r C(r, a) T(r, c) for a <= c, c in [c0, c1)
-    const int nab = (c1 - 1) / 16 + 1;         // row blocks that touch a <= c
-    double* __restrict__ dst = A.dst + dst_off;
-    for (int ab = w; ab < nab; ab += NW) {
-        f64x4 o[K::NCB];
-        bool act[K::NCB];
-#pragma unroll
-        for (int j = 0; j < K::NCB; ++j) {
-            o[j] = f64x4{0.0, 0.0, 0.0, 0.0};
-            const int cbeg = c0 + j * 16;
-            act[j] = (cbeg < c1) && (cbeg + 15 >= ab * 16);
-        }
-        const double* ccol = cr + ab * 16 + lm;
-        double an[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) an[u] = ccol[(int64_t)(u * 4 + lg) * ldm];
-        for (int r0 = 0; r0 < np; r0 += 16) {
-            double ac[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) ac[u] = an[u];
-            if (r0 + 16 < np) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) an[u] = ccol[(int64_t)(r0 + 16 + u * 4 + lg) * ldm];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-#pragma unroll
-                for (int j = 0; j < K::NCB; ++j) {
-                    if (act[j]) {
-                        const double bt = ts[(r0 + u * 4 + lg) * K::LDT + j * 16 + lm];
-                        o[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u], bt, o[j], 0, 0, 0);
+                for (int rb = 0; rb < NRB; ++rb) {
+                    if (rb < nrb) {
+                        const double af = zs[(k4 + lg) * LDZ + rb * 16 + lm];
+                        acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, acc[rb], 0, 0, 0);
                     }
                 }
             }
         }
+        if (more) commit((ch + 1) * KC, buf[(ch + 1) & 1]);
+        __syncthreads();
+    }
+
+    // ---------------------------------------------------------------- Q2: O^T = T^T C
+    // C column blocks (16 columns a, all rows) staged through LDS and shared by the
+    // waves; wave w needs a <= its last column, so it stops after block cw/16.
+    double qst[K::EPT_Q];
+    auto fetch_q = [&](int ab) {
 #pragma unroll
-        for (int j = 0; j < K::NCB; ++j) {
-            if (!act[j]) continue;
-            const int c = c0 + j * 16 + lm;
-            if (c >= c1) continue;
+        for (int t = 0; t < K::EPT_Q; ++t) {
+            const int e = tid + t * K::NT;
+            const int r = e >> 4, a = e & 15;
+            const bool ok = r < np;
+            const double v = cr[ok ? (int64_t)r * ldm + ab * 16 + a : 0];
+            qst[t] = ok ? v : 0.0;
+        }
+    };
+    auto commit_q = [&](double* bp) {
+#pragma unroll
+        for (int t = 0; t < K::EPT_Q; ++t) {
+            const int e = tid + t * K::NT;
+            if ((e >> 4) < np) bp[(e >> 4) * K::LDQ + (e & 15)] = qst[t];
+        }
+    };
+    const int nab = (c1 - 1) / 16 + 1;
+    const int my_last_ab = wave_active ? min(nab - 1, cw / 16) : -1;
+    double* __restrict__ dst = A.dst + dst_off;
+    double* qbuf[2] = {smem, smem + np * K::LDQ};
+    fetch_q(0);
+    commit_q(qbuf[0]);
+    __syncthreads();
+    for (int ab = 0; ab < nab; ++ab) {
+        const bool more = (ab + 1) < nab;
+        if (more) fetch_q(ab + 1);
+        if (ab <= my_last_ab) {
+            const double* cs = qbuf[ab & 1] + lg * K::LDQ + lm;
+            f64x4 o0 = f64x4{0.0, 0.0, 0.0, 0.0}, o1 = o0;
+#pragma unroll
+            for (int rb = 0; rb < NRB; ++rb) {
+                if (rb < nrb) {
+                    const double* crow = cs + rb * 16 * K::LDQ;
+                    o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][0], crow[0], o0, 0, 0, 0);
+                    o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][1], crow[4 * K::LDQ], o1, 0, 0, 0);
+                    o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][2], crow[8 * K::LDQ], o0, 0, 0, 0);
+                    o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][3], crow[12 * K::LDQ], o1, 0, 0, 0);
+                }
+            }
+            // lane (lm, lg), element q: O(a = 16 ab + lm, c = cw + lg + 4q)
+            const int a = ab * 16 + lm;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int a = ab * 16 + lg + 4 * q;
-                const int64_t p = pair0(c, a);
-                if (a <= c && p < limit) dst[p] = o[j][q];
+                const int c = cw + lg + 4 * q;
+                if (c < c1 && a <= c) {
+                    const int64_t pp = pair0(c, a);
+                    if (pp < limit) dst[pp] = o0[q] + o1[q];
+                }
             }
         }
+        if (more) commit_q(qbuf[(ab + 1) & 1]);
+        __syncthreads();
     }
 }
 
-template <int LT, int NW>
-static hipError_t launch_lt(const HalfArgs& a, hipStream_t st) {
-    using K = HalfCfg<LT, NW>;
+template <int NW, int NRB, int KC, int MINB>
+static hipError_t launch_v(const HalfArgs& a, hipStream_t st) {
+    using K = HalfCfg<NW, NRB, KC>;
     const int ldz = a.np + 2;
-    const size_t panel = (size_t)(KC * ldz + KC * K::LDC);
-    const size_t tsz = (size_t)a.np * K::LDT;
-    const size_t shmem = sizeof(double) * (2 * panel > tsz ? 2 * panel : tsz);
+    const size_t q1 = 2 * (size_t)(KC * ldz + KC * K::LDC);
+    const size_t q2 = 2 * (size_t)a.np * K::LDQ;
+    const size_t shmem = sizeof(double) * (q1 > q2 ? q1 : q2);
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_half_fp64<LT, NW>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_half_fp64<NW, NRB, KC, MINB>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     int64_t g = (a.ntile_total + 7) / 8 * 8;
     if (g > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_half_fp64<LT, NW>), dim3((unsigned)g), dim3(K::NT), shmem, st, a);
+    hipLaunchKernelGGL((k_half_fp64<NW, NRB, KC, MINB>), dim3((unsigned)g), dim3(K::NT), shmem, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_half_fp64(const HalfArgs& a, int lt, hipStream_t st) {
-    if (a.np > 256 || (a.np & 15) || (a.mp & 15)) return hipErrorInvalidValue;
-    switch (lt) {
-        case 16: return launch_lt<16, 4>(a, st);
-        case 32: return launch_lt<32, 4>(a, st);
-        case 64: return launch_lt<64, 8>(a, st);
+// Tile variants: (waves per work-group, Q1 panel depth, work-groups per CU hint).
+// Columns per work-group = 16 x waves.
+const HalfVariant kHalfVariants[] = {
+    {4, 8, 1}, {4, 16, 1}, {8, 8, 1}, {8, 16, 1}, {2, 16, 1}, {4, 8, 2}, {2, 8, 2},
+};
+const int kNumHalfVariants = sizeof(kHalfVariants) / sizeof(kHalfVariants[0]);
+
+template <int NRB>
+static hipError_t launch_rows(const HalfArgs& a, int v, hipStream_t st) {
+    switch (v) {
+        case 0: return launch_v<4, NRB, 8, 1>(a, st);
+        case 1: return launch_v<4, NRB, 16, 1>(a, st);
+        case 2: return launch_v<8, NRB, 8, 1>(a, st);
+        case 3: return launch_v<8, NRB, 16, 1>(a, st);
+        case 4: return launch_v<2, NRB, 16, 1>(a, st);
+        case 5: return launch_v<4, NRB, 8, 2>(a, st);
+        case 6: return launch_v<2, NRB, 8, 2>(a, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_half_fp64(const HalfArgs& a, int variant, hipStream_t st) {
+    if (a.np > 256 || (a.np & 15) || (a.mp & 15)) return hipErrorInvalidValue;
+    const int nrb = a.np / 16;
+    if (nrb <= 4) return launch_rows<4>(a, variant, st);
+    if (nrb <= 8) return launch_rows<8>(a, variant, st);
+    if (nrb <= 12) return launch_rows<12>(a, variant, st);
+    if (nrb <= 15) return launch_rows<15>(a, variant, st);
+    return launch_rows<16>(a, variant, st);
 }
 
 }  // namespace cipsi

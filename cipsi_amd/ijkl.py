@@ -22,7 +22,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcipsi_ijkl.so")
+# CIPSI_IJKL_LIBRARY selects another build of the library (same ABI)
+LIB_PATH = os.environ.get("CIPSI_IJKL_LIBRARY") or os.path.join(_HERE, "libcipsi_ijkl.so")
 
 PREC_FP64 = 0
 PREC_SPLIT_FP16 = 1
