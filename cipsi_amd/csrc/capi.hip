@@ -89,7 +89,9 @@ int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int6
     return CIPSI_OK;
 }
 
-int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half) {
+int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half, int n, int m, int precision) {
+    auto it = ctx->plans.find(std::make_tuple(n, m, precision));
+    if (it != ctx->plans.end()) return half == 1 ? it->second.first : it->second.second;
     return half == 1 ? ctx->var_half1 : ctx->var_half2;
 }
 
@@ -106,6 +108,12 @@ int run_pad_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_c, int ldc, dou
 
 int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, const double* cr,
              double* dst, int precision, int n_slices) {
+    return run_half_variant(ctx, half, n, m, src, cr, dst, precision, n_slices,
+                            ctx_half_variant(ctx, half, n, m, precision));
+}
+
+int run_half_variant(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src,
+                     const double* cr, double* dst, int precision, int n_slices, int var) {
     const int64_t pa = npairs(n), pm = npairs(m);
     HalfArgs a{};
     a.src = src;
@@ -119,7 +127,6 @@ int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, con
     a.mp = round16(m);
     a.half = half;
     a.b_begin = 0;
-    const int var = ctx_half_variant(ctx, half);
     const int lt = variant_columns(var);
     if (half == 1) {
         a.nb = pa;

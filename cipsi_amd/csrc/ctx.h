@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -31,9 +32,10 @@ struct cipsi_ijkl_ctx {
     DevBuf slices;   // split-FP16 operand slices
     int ts_m = -1, ts_lt = -1;
     int64_t ts_total = 0;
-    // tile plans (kernel variant index) for the two halves
+    // tile plans (kernel variant index) for the two halves: defaults and per-size plans
     int var_half1 = 3;
     int var_half2 = 3;
+    std::map<std::tuple<int, int, int>, std::pair<int, int>> plans;
     // per-stage event timing
     bool timing = false;
     struct Pending { std::string stage; hipEvent_t a, b; };
@@ -65,6 +67,9 @@ int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int6
 int run_pad_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_c, int ldc, double** cr);
 int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, const double* cr,
              double* dst, int precision, int n_slices);
+int run_half_variant(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src,
+                     const double* cr, double* dst, int precision, int n_slices, int var);
+int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half, int n, int m, int precision);
 
 // split-FP16 path (half_split.hip)
 bool split_supported(int n_slices);

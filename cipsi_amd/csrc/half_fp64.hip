@@ -32,7 +32,7 @@ struct HalfCfg {
     static constexpr int NU = ((NRB + PPS - 1) / PPS) * SUB; // loads per thread per panel
     static constexpr int EPT_C = (KC * LT + NT - 1) / NT;    // C panel elements per thread
     static constexpr int LDQ = 18;                           // Q2 C-block row stride
-    static constexpr int EPT_Q = (NRB * 16 * 16 + NT - 1) / NT;
+    static constexpr int EPT_Q = (NRB * 16 * 16 + NT - 1) / NT;   // rows padded to NRB*16
 };
 
 __device__ inline void work_item(const HalfArgs& A, int lt, int64_t item, int64_t& b, int& tile,
@@ -69,7 +69,8 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
     const int lane = tid & 63, w = tid >> 6;
     const int lm = lane & 15, lg = lane >> 4;
     const int np = A.np, n = A.n;
-    const int LDZ = np + 2;
+    constexpr int RP = NRB * 16;                   // rows held in LDS (zero beyond n)
+    constexpr int LDZ = RP + 2;
 
     // XCD-aware mapping: consecutive work items (same matrix, neighbouring tiles)
     // land on the same XCD so the Z_b row is fetched into one L2.
@@ -88,8 +89,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
     const int ldm = A.ldm;
     const int nrb = np >> 4;
 
-    const int panel = KC * LDZ + KC * K::LDC;
-    double* buf[2] = {smem, smem + panel};
+    constexpr int PANEL = KC * LDZ + KC * K::LDC;   // doubles per Q1 stage buffer
 
     // ---------------------------------------------------------------- Q1 staging
     // Panel = Z(r, s0 + kk) for r < np, kk < KC, taken in 16-row passes.  Below the
@@ -116,34 +116,34 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
             zslot(u, p, e);
             zmap(p, e, s0, r, kk);
             const int s = s0 + kk;
-            const bool ok = (p < nrb) && (r < n) && (s < n);
+            const bool ok = (r < n) && (s < n);
             const int hi = r >= s ? r : s, lo = r >= s ? s : r;
             const int idx = ok ? ((hi * (hi + 1)) >> 1) + lo : 0;
-            const double v = zsrc[idx];
-            zst[u] = ok ? v : 0.0;
+            zst[u] = zsrc[idx];            // masked at commit: no wait on the load here
         }
 #pragma unroll
         for (int t = 0; t < K::EPT_C; ++t) {
             const int e = tid + t * K::NT;
             const int kk = e / K::LT, c = e % K::LT;
-            const bool ok = (e < KC * K::LT) && (c0 + c < A.mp);
-            const double v = cr[ok ? (int64_t)(s0 + kk) * ldm + c0 + c : 0];
-            cst[t] = ok ? v : 0.0;
+            const bool ok = (e < KC * K::LT) && (c0 + c < A.mp) && (s0 + kk < np);
+            cst[t] = cr[ok ? (int64_t)(s0 + kk) * ldm + c0 + c : 0];
         }
     };
-    auto commit = [&](int s0, double* bp) {
+    auto commit = [&](int s0, int boff) {
 #pragma unroll
         for (int u = 0; u < K::NU; ++u) {
             int p, e, r, kk;
             zslot(u, p, e);
             zmap(p, e, s0, r, kk);
-            if (p < nrb) bp[kk * LDZ + r] = zst[u];
+            const bool ok = (r < n) && (s0 + kk < n);
+            if (p < NRB) smem[boff + kk * LDZ + r] = ok ? zst[u] : 0.0;
         }
-        double* cs = bp + KC * LDZ;
 #pragma unroll
         for (int t = 0; t < K::EPT_C; ++t) {
             const int e = tid + t * K::NT;
-            if (e < KC * K::LT) cs[(e / K::LT) * K::LDC + (e % K::LT)] = cst[t];
+            const int kk = e / K::LT, c = e % K::LT;
+            const bool ok = (c0 + c < A.mp) && (s0 + kk < np);
+            if (e < KC * K::LT) smem[boff + KC * LDZ + kk * K::LDC + c] = ok ? cst[t] : 0.0;
         }
     };
 
@@ -152,29 +152,28 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
 #pragma unroll
     for (int i = 0; i < NRB; ++i) acc[i] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    const int nchunk = np / KC;
+    const int nchunk = (np + KC - 1) / KC;
     fetch(0);
-    commit(0, buf[0]);
+    commit(0, 0);
     __syncthreads();
     for (int ch = 0; ch < nchunk; ++ch) {
         const bool more = (ch + 1) < nchunk;
         if (more) fetch((ch + 1) * KC);
         if (wave_active) {
-            const double* zs = buf[ch & 1];
-            const double* cs = zs + KC * LDZ;
+            const int zo = (ch & 1) * PANEL;
+            const int co = zo + KC * LDZ;
 #pragma unroll
             for (int k4 = 0; k4 < KC; k4 += 4) {
-                const double bf = cs[(k4 + lg) * K::LDC + 16 * w + lm];
+                const double bf = smem[co + (k4 + lg) * K::LDC + 16 * w + lm];
 #pragma unroll
                 for (int rb = 0; rb < NRB; ++rb) {
-                    if (rb < nrb) {
-                        const double af = zs[(k4 + lg) * LDZ + rb * 16 + lm];
-                        acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, acc[rb], 0, 0, 0);
-                    }
+                    const double af = smem[zo + (k4 + lg) * LDZ + rb * 16 + lm];
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, acc[rb], 0, 0, 0);
                 }
             }
         }
-        if (more) commit((ch + 1) * KC, buf[(ch + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (more) commit((ch + 1) * KC, ((ch + 1) & 1) * PANEL);
         __syncthreads();
     }
 
@@ -188,39 +187,36 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
             const int e = tid + t * K::NT;
             const int r = e >> 4, a = e & 15;
             const bool ok = r < np;
-            const double v = cr[ok ? (int64_t)r * ldm + ab * 16 + a : 0];
-            qst[t] = ok ? v : 0.0;
+            qst[t] = cr[ok ? (int64_t)r * ldm + ab * 16 + a : 0];
         }
     };
-    auto commit_q = [&](double* bp) {
+    constexpr int QBUF = RP * K::LDQ;               // doubles per Q2 stage buffer
+    auto commit_q = [&](int boff) {
 #pragma unroll
         for (int t = 0; t < K::EPT_Q; ++t) {
             const int e = tid + t * K::NT;
-            if ((e >> 4) < np) bp[(e >> 4) * K::LDQ + (e & 15)] = qst[t];
+            if ((e >> 4) < RP) smem[boff + (e >> 4) * K::LDQ + (e & 15)] = (e >> 4) < np ? qst[t] : 0.0;
         }
     };
     const int nab = (c1 - 1) / 16 + 1;
     const int my_last_ab = wave_active ? min(nab - 1, cw / 16) : -1;
     double* __restrict__ dst = A.dst + dst_off;
-    double* qbuf[2] = {smem, smem + np * K::LDQ};
     fetch_q(0);
-    commit_q(qbuf[0]);
+    commit_q(0);
     __syncthreads();
     for (int ab = 0; ab < nab; ++ab) {
         const bool more = (ab + 1) < nab;
         if (more) fetch_q(ab + 1);
         if (ab <= my_last_ab) {
-            const double* cs = qbuf[ab & 1] + lg * K::LDQ + lm;
+            const int qo = (ab & 1) * QBUF + lg * K::LDQ + lm;
             f64x4 o0 = f64x4{0.0, 0.0, 0.0, 0.0}, o1 = o0;
 #pragma unroll
             for (int rb = 0; rb < NRB; ++rb) {
-                if (rb < nrb) {
-                    const double* crow = cs + rb * 16 * K::LDQ;
-                    o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][0], crow[0], o0, 0, 0, 0);
-                    o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][1], crow[4 * K::LDQ], o1, 0, 0, 0);
-                    o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][2], crow[8 * K::LDQ], o0, 0, 0, 0);
-                    o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][3], crow[12 * K::LDQ], o1, 0, 0, 0);
-                }
+                const int ro = qo + rb * 16 * K::LDQ;
+                o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][0], smem[ro], o0, 0, 0, 0);
+                o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][1], smem[ro + 4 * K::LDQ], o1, 0, 0, 0);
+                o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][2], smem[ro + 8 * K::LDQ], o0, 0, 0, 0);
+                o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][3], smem[ro + 12 * K::LDQ], o1, 0, 0, 0);
             }
             // lane (lm, lg), element q: O(a = 16 ab + lm, c = cw + lg + 4q)
             const int a = ab * 16 + lm;
@@ -233,7 +229,8 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
                 }
             }
         }
-        if (more) commit_q(qbuf[(ab + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (more) commit_q(((ab + 1) & 1) * QBUF);
         __syncthreads();
     }
 }
@@ -241,9 +238,9 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
 template <int NW, int NRB, int KC, int MINB>
 static hipError_t launch_v(const HalfArgs& a, hipStream_t st) {
     using K = HalfCfg<NW, NRB, KC>;
-    const int ldz = a.np + 2;
+    const int ldz = NRB * 16 + 2;
     const size_t q1 = 2 * (size_t)(KC * ldz + KC * K::LDC);
-    const size_t q2 = 2 * (size_t)a.np * K::LDQ;
+    const size_t q2 = 2 * (size_t)NRB * 16 * K::LDQ;
     const size_t shmem = sizeof(double) * (q1 > q2 ? q1 : q2);
     static bool attr_set = false;
     if (!attr_set) {

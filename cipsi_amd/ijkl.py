@@ -42,7 +42,8 @@ EXPORTS = [
     "cipsi_ijkl_transform_packed_dev", "cipsi_ijkl_transform_labels_dev",
     "cipsi_ijkl_unpack_labels_dev", "cipsi_ijkl_half1_dev", "cipsi_ijkl_half2_dev",
     "cipsi_ijkl_tmono", "cipsi_ijkl_enable_timing", "cipsi_ijkl_read_timing",
-    "cipsi_ijkl_reset_timing",
+    "cipsi_ijkl_reset_timing", "cipsi_ijkl_autotune", "cipsi_ijkl_set_plan", "cipsi_ijkl_get_plan",
+    "cipsi_ijkl_num_variants", "cipsi_ijkl_variant_info",
 ]
 
 
@@ -95,6 +96,12 @@ def load_library(path: str = LIB_PATH):
     L.cipsi_ijkl_enable_timing.argtypes = [_vp, _c_int]
     L.cipsi_ijkl_read_timing.argtypes = [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(_c_i64)]
     L.cipsi_ijkl_reset_timing.argtypes = [_vp]
+    _ip = ctypes.POINTER(_c_int)
+    L.cipsi_ijkl_autotune.argtypes = [_vp, _c_int, _c_int, _c_int, _ip, _ip]
+    L.cipsi_ijkl_set_plan.argtypes = [_vp, _c_int, _c_int, _c_int, _c_int, _c_int]
+    L.cipsi_ijkl_get_plan.argtypes = [_vp, _c_int, _c_int, _c_int, _ip, _ip]
+    L.cipsi_ijkl_num_variants.argtypes = []
+    L.cipsi_ijkl_variant_info.argtypes = [_c_int, _ip, _ip, _ip]
     _lib = L
     return L
 
@@ -251,6 +258,24 @@ class Context:
             "half2_dev")
         return out
 
+    # --------------------------------------------------------------- tile plans
+    def autotune(self, n_ao, n_mo, precision="fp64"):
+        """Time every tile variant of both halves at this size; returns (half1, half2)."""
+        v1, v2 = _c_int(-1), _c_int(-1)
+        self._check(self.lib.cipsi_ijkl_autotune(self.handle, n_ao, n_mo, _PREC[precision],
+                                                 ctypes.byref(v1), ctypes.byref(v2)), "autotune")
+        return v1.value, v2.value
+
+    def set_plan(self, n_ao, n_mo, half1=-1, half2=-1, precision="fp64"):
+        self._check(self.lib.cipsi_ijkl_set_plan(self.handle, n_ao, n_mo, _PREC[precision],
+                                                 half1, half2), "set_plan")
+
+    def get_plan(self, n_ao, n_mo, precision="fp64"):
+        v1, v2 = _c_int(-1), _c_int(-1)
+        self._check(self.lib.cipsi_ijkl_get_plan(self.handle, n_ao, n_mo, _PREC[precision],
+                                                 ctypes.byref(v1), ctypes.byref(v2)), "get_plan")
+        return v1.value, v2.value
+
     # --------------------------------------------------------------- timing
     def enable_timing(self, on=True):
         self._check(self.lib.cipsi_ijkl_enable_timing(self.handle, int(on)), "enable_timing")
@@ -275,6 +300,17 @@ class Context:
         self._check(self.lib.cipsi_ijkl_tmono(self.handle, n, m, _ptr(hp), _ptr(cm), ldc,
                                               _ptr(out)), "tmono")
         return out
+
+
+def variants():
+    """[(waves per work-group, panel depth, work-groups per CU hint)] of the built kernels."""
+    L = load_library()
+    out = []
+    for v in range(L.cipsi_ijkl_num_variants()):
+        w, k, b = _c_int(), _c_int(), _c_int()
+        L.cipsi_ijkl_variant_info(v, ctypes.byref(w), ctypes.byref(k), ctypes.byref(b))
+        out.append((w.value, k.value, b.value))
+    return out
 
 
 def workspace_bytes(n_ao, n_mo, precision="fp64", with_labels=False):

@@ -115,6 +115,23 @@ int cipsi_ijkl_half2_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *
 int cipsi_ijkl_tmono(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *h_packed,
                      const double *c, int ldc, double *h_mo_packed);
 
+/* Tile autotuner: times every built tile variant of the two half-transform kernels
+   on device-resident synthetic data of this size and keeps the fastest per half in the
+   context (used by every later call with the same n_ao, n_mo, precision).  best_half1 /
+   best_half2 (may be NULL) receive the chosen variant indices.  No counterpart in the
+   reference; tile shapes are specific to this implementation. */
+int cipsi_ijkl_autotune(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, int precision,
+                        int *best_half1, int *best_half2);
+/* Set / read the plan (variant indices) used for a size; -1 leaves a half unchanged. */
+int cipsi_ijkl_set_plan(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, int precision, int var_half1,
+                        int var_half2);
+int cipsi_ijkl_get_plan(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, int precision, int *var_half1,
+                        int *var_half2);
+/* Number of built tile variants, and the shape of one (waves per work-group, panel depth,
+   work-groups per CU hint). */
+int cipsi_ijkl_num_variants(void);
+int cipsi_ijkl_variant_info(int variant, int *waves, int *panel_depth, int *min_blocks);
+
 /* Per-kernel GPU timing with HIP events recorded on the context stream around each
    launch of the named stage ("expand", "half1", "transpose", "half2", "pad_c",
    "unpack").  For benchmarking; no counterpart in the reference (whose timer calls
