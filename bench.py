@@ -31,6 +31,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MO integrals/s at N_AO=240: FP64 MFMA vs split-FP16 at <=1e-10 abs err"
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this workload (fetched = 2.0 x FETCH_SIZE, written = 1.0 x
+# WRITE_SIZE, factors from a 1 GiB copy in the same process); keyed by
+# (n_ao, n_mo, precision).  Summaries: profiles/fp64_n240_kernels.md.
+MEASURED_TRAFFIC = {(240, 240, "fp64"): (2.0 * 3924450.4 + 6780965.6) * 1024}
 FP16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 
@@ -142,7 +147,8 @@ def main():
     achieved = f1 / (stages["half1"] * 1e-3) / 1e12
     roofline = {"bound": "mfma", "kernel": "k_half_fp64 (half 1: Q1+Q2)",
                 "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                "traffic": MEASURED_TRAFFIC.get((n, m, prec)),
                 "stage_ms": {k: round(v, 4) for k, v in stages.items()},
                 "half2_tflops": round(flops_half2(n, m) / (stages["half2"] * 1e-3) / 1e12, 3)
                 if stages["half2"] else None}
