@@ -231,6 +231,9 @@ int cipsi_ijkl_release_workspace(cipsi_ijkl_ctx* ctx) {
     ctx_free(ctx->scal);
     ctx_free(ctx->tile_start);
     ctx_free(ctx->slices);
+    ctx_free(ctx->adj_g);
+    ctx_free(ctx->adj_gh);
+    ctx_free(ctx->adj_small);
     ctx->ts_m = -1;
     return CIPSI_OK;
 }
@@ -239,6 +242,7 @@ int cipsi_ijkl_destroy(cipsi_ijkl_ctx* ctx) {
     if (!ctx) return CIPSI_ERR_ARG;
     cipsi_ijkl_reset_timing(ctx);
     cipsi_ijkl_release_workspace(ctx);
+    ctx_rocblas_destroy(ctx);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return CIPSI_OK;

@@ -30,6 +30,10 @@ struct cipsi_ijkl_ctx {
     DevBuf io_in, io_in2, io_out, io_c;   // staging for host-pointer entry points
     DevBuf tile_start;                    // half-2 work table
     DevBuf slices;   // split-FP16 operand slices
+    DevBuf adj_g;    // energy/gradient: packed MO integrals
+    DevBuf adj_gh;   // energy/gradient: scaled pair density Gh[b][a] = m(b) Gamma(a,b)
+    DevBuf adj_small;  // energy/gradient: h_MO, hC, reduction partials
+    void* blas = nullptr;  // rocblas_handle, created on first gradient call
     int ts_m = -1, ts_lt = -1;
     int64_t ts_total = 0;
     // tile plans (kernel variant index) for the two halves: defaults and per-size plans
@@ -70,6 +74,9 @@ int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, con
 int run_half_variant(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src,
                      const double* cr, double* dst, int precision, int n_slices, int var);
 int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half, int n, int m, int precision);
+
+// energy / gradient (adjoint.hip)
+void ctx_rocblas_destroy(cipsi_ijkl_ctx* ctx);
 
 // split-FP16 path (half_split.hip)
 bool split_supported(int n_slices);

@@ -66,6 +66,17 @@ module cipsi_ijkl_mod
       real(c_double), intent(in) :: h_packed(*), c(*)
       real(c_double), intent(out) :: h_mo(*)
     end function
+
+    ! E(C) = sum h^MO gamma + 1/2 sum Gamma (pq|rs)_MO and dE/dC (n_ao x n_mo).
+    ! gamma is n_mo x n_mo; Gamma_packed is packed like the MO stream.
+    integer(c_int) function cipsi_ijkl_energy_grad(ctx, n_ao, n_mo, ao_packed, h_packed, c, &
+        ldc, gamma, Gamma_packed, energy, grad) bind(C, name="cipsi_ijkl_energy_grad")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n_ao, n_mo, ldc
+      real(c_double), intent(in) :: ao_packed(*), h_packed(*), c(*), gamma(*), Gamma_packed(*)
+      real(c_double), intent(out) :: energy, grad(*)
+    end function
   end interface
 
 contains

@@ -44,6 +44,7 @@ EXPORTS = [
     "cipsi_ijkl_tmono", "cipsi_ijkl_enable_timing", "cipsi_ijkl_read_timing",
     "cipsi_ijkl_reset_timing", "cipsi_ijkl_autotune", "cipsi_ijkl_set_plan", "cipsi_ijkl_get_plan",
     "cipsi_ijkl_num_variants", "cipsi_ijkl_variant_info",
+    "cipsi_ijkl_energy_grad", "cipsi_ijkl_energy_grad_dev",
 ]
 
 
@@ -102,6 +103,10 @@ def load_library(path: str = LIB_PATH):
     L.cipsi_ijkl_get_plan.argtypes = [_vp, _c_int, _c_int, _c_int, _ip, _ip]
     L.cipsi_ijkl_num_variants.argtypes = []
     L.cipsi_ijkl_variant_info.argtypes = [_c_int, _ip, _ip, _ip]
+    L.cipsi_ijkl_energy_grad.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp,
+                                         _vp, _vp]
+    L.cipsi_ijkl_energy_grad_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp,
+                                             _vp, _vp, _vp]
     _lib = L
     return L
 
@@ -300,6 +305,45 @@ class Context:
         self._check(self.lib.cipsi_ijkl_tmono(self.handle, n, m, _ptr(hp), _ptr(cm), ldc,
                                               _ptr(out)), "tmono")
         return out
+
+    def energy_grad(self, ao_packed, h_packed, c, gamma, Gamma_packed, want_grad=True):
+        """E(C) = sum h^MO gamma + 1/2 sum Gamma (pq|rs)_MO and dE/dC (N x M).
+
+        gamma is M x M symmetric; Gamma_packed is the pair density packed like the MO
+        stream.  Torch CUDA tensors run on torch's current stream and return tensors;
+        numpy arrays use the synchronous host entry point.
+        """
+        n, m = c.shape
+        if _is_torch(ao_packed):
+            import torch
+            dev = ao_packed.device
+            cm, ldc = self._cmat(c)
+            args = [x.contiguous() for x in (ao_packed, h_packed, gamma, Gamma_packed)]
+            for x, size in zip(args, (n_pairs(n_pairs(n)), n_pairs(n), m * m,
+                                      n_pairs(n_pairs(m)))):
+                if x.dtype != torch.float64 or not x.is_cuda or x.numel() != size:
+                    raise ValueError("energy_grad: inputs must be float64 CUDA tensors of the "
+                                     "documented sizes")
+            e = torch.empty(1, dtype=torch.float64, device=dev)
+            g = torch.empty((m, n), dtype=torch.float64, device=dev) if want_grad else None
+            self._use_torch_stream()
+            self._check(self.lib.cipsi_ijkl_energy_grad_dev(
+                self.handle, n, m, _ptr(args[0]), _ptr(args[1]), _ptr(cm), ldc, _ptr(args[2]),
+                _ptr(args[3]), _ptr(e), _ptr(g) if want_grad else None), "energy_grad")
+            return e[0], (g.t() if want_grad else None)
+        cm, ldc = self._cmat(c)
+        arrs = [np.ascontiguousarray(x, dtype=np.float64)
+                for x in (ao_packed, h_packed, gamma, Gamma_packed)]
+        for x, size in zip(arrs, (n_pairs(n_pairs(n)), n_pairs(n), m * m, n_pairs(n_pairs(m)))):
+            if x.size != size:
+                raise ValueError("energy_grad: input of the wrong size")
+        e = ctypes.c_double()
+        g = np.zeros((n, m), order="F") if want_grad else None
+        self.lib.cipsi_ijkl_reset_stream(self.handle)
+        self._check(self.lib.cipsi_ijkl_energy_grad(
+            self.handle, n, m, _ptr(arrs[0]), _ptr(arrs[1]), _ptr(cm), ldc, _ptr(arrs[2]),
+            _ptr(arrs[3]), ctypes.byref(e), _ptr(g) if want_grad else None), "energy_grad")
+        return e.value, g
 
 
 def variants():

@@ -107,6 +107,16 @@ def psd_trace1(n: int, seed: int, rank: int | None = None) -> np.ndarray:
     return a / np.trace(a)
 
 
+def pair_density(n_mo: int, seed: int):
+    """(gamma, Gamma_packed): a unit-trace PSD one-particle density (M x M, row-major)
+    and a seeded symmetric pair density on the MO pair supermatrix, packed like the MO
+    stream (P_M(P_M+1)/2 doubles, entries ~ N(0, 1) / P_M)."""
+    pm = n_pairs(n_mo)
+    gamma = psd_trace1(n_mo, seed)
+    rng = np.random.default_rng(seed + 1)
+    return gamma, rng.standard_normal(n_pairs(pm)) / pm
+
+
 def packed_lower(a: np.ndarray) -> np.ndarray:
     return a[np.tril_indices(a.shape[0])].copy()
 

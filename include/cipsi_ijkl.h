@@ -115,6 +115,29 @@ int cipsi_ijkl_half2_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *
 int cipsi_ijkl_tmono(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *h_packed,
                      const double *c, int ldc, double *h_mo_packed);
 
+/* Energy and its gradient with respect to the MO coefficients (the adjoint of the
+ * transform).  The reference has no such entry point: its consumers (fock.f:80-101,
+ * and the CI energy assembled downstream) evaluate the same contraction
+ *   E(C) = sum_pq h^MO_pq gamma_pq + 1/2 sum_pqrs Gamma_pqrs (pq|rs)_MO
+ * on the tmono / tijkld outputs; this computes E and dE/dC on the device from the
+ * AO inputs, recomputing the half-transformed integrals instead of storing them.
+ *   gamma        M x M symmetric one-particle density (row-major)
+ *   Gamma_packed P_M(P_M+1)/2 pair density, packed like mo_packed:
+ *                Gamma_pqrs = Gamma_packed[pair0(max(a,b),min(a,b))], a = pair(p,q),
+ *                b = pair(r,s), for every index order
+ *   grad         N_AO x N_MO column-major (leading dimension N_AO); may be NULL
+ *                (energy only).
+ * FP64 throughout; one rocBLAS DGEMM (P_A x P_M x P_M) forms the pair-density
+ * contraction.  Workspace adds P_M(P_M+1)/2 + P_M^2 doubles to the transform's. */
+int cipsi_ijkl_energy_grad(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *ao_packed,
+                           const double *h_packed, const double *c, int ldc,
+                           const double *gamma, const double *Gamma_packed, double *energy,
+                           double *grad);
+int cipsi_ijkl_energy_grad_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo,
+                               const double *d_ao_packed, const double *d_h_packed,
+                               const double *d_c, int ldc, const double *d_gamma,
+                               const double *d_Gamma_packed, double *d_energy, double *d_grad);
+
 /* Tile autotuner: times every built tile variant of the two half-transform kernels
    on device-resident synthetic data of this size and keeps the fastest per half in the
    context (used by every later call with the same n_ao, n_mo, precision).  best_half1 /

@@ -1,0 +1,158 @@
+"""Adjoint of the transform: E(C) and dE/dC from cipsi_ijkl_energy_grad[_dev].
+
+* small sizes: energy and gradient against the oracle's serial FP64 or_energy_grad
+  (itself pinned by tests/test_oracle_energy.py);
+* N_AO = 160, N_MO = 100 (rank-480 synthetic integrals): the gradient, taken through a
+  torch.autograd.Function whose backward returns the library's dE/dC, against central
+  finite differences (step 1e-6) of the library's own energy along seeded directions,
+  and against an FP64 CPU gradient built from the oracle's tpqkld rows for sampled AO
+  indices mu; tolerance 1e-8 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cipsi_amd import synth
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-8
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from cipsi_amd import ijkl
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = ijkl.Context(0)
+    yield c
+    c.close()
+
+
+class TransformEnergy(torch.autograd.Function):
+    """E(C) through the library; backward returns the library's adjoint dE/dC."""
+
+    @staticmethod
+    def forward(fctx, c, ao, h, gamma, Gam, lib_ctx):
+        e, g = lib_ctx.energy_grad(ao, h, c, gamma, Gam)
+        fctx.save_for_backward(g)
+        return e
+
+    @staticmethod
+    def backward(fctx, ge):
+        (g,) = fctx.saved_tensors
+        return ge * g, None, None, None, None, None
+
+
+def _inputs(n, m, seed, rank=None, decay=4.0):
+    B = synth.eri_factors(n, rank or 2 * n, decay, seed)
+    rng = np.random.default_rng(seed + 7)
+    c = np.asfortranarray(synth.orthonormal(n, seed + 1)[:, :m]
+                          + 0.05 * rng.standard_normal((n, m)))
+    h = synth.symmetric_packed(n, seed + 3)
+    gamma, Gam = synth.pair_density(m, seed + 5)
+    return B, c, h, gamma, Gam
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(np.asarray(b))))
+
+
+@pytest.mark.parametrize("n,m", [(12, 9), (24, 20), (40, 40)])
+def test_energy_grad_matches_oracle(ctx, n, m):
+    print(f"[adjoint-oracle] n={n} m={m}", flush=True)
+    B, c, h, gamma, Gam = _inputs(n, m, 300 + n)
+    V = synth.eri_packed(B)
+    e_ref, g_ref = oracle.energy_grad(V, h, c, gamma, Gam)
+    # host entry point
+    e, g = ctx.energy_grad(V, h, c, gamma, Gam)
+    assert abs(e - e_ref) <= 1e-12 * max(1.0, abs(e_ref))
+    assert _rel(g, g_ref) <= 1e-11
+    # device entry point on torch's stream
+    dev = lambda x: torch.as_tensor(np.ascontiguousarray(x), device="cuda")
+    e_d, g_d = ctx.energy_grad(dev(V), dev(h), dev(c), dev(gamma), dev(Gam))
+    assert abs(float(e_d) - e_ref) <= 1e-12 * max(1.0, abs(e_ref))
+    assert _rel(g_d.cpu().numpy(), g_ref) <= 1e-11
+    # energy only
+    e_only, none = ctx.energy_grad(V, h, c, gamma, Gam, want_grad=False)
+    assert none is None and e_only == e
+
+
+def test_energy_grad_rejects_bad_input(ctx):
+    from cipsi_amd import ijkl
+    B, c, h, gamma, Gam = _inputs(8, 6, 5)
+    V = synth.eri_packed(B)
+    with pytest.raises(ValueError):
+        ctx.energy_grad(V[:-1], h, c, gamma, Gam)
+    L = ijkl.load_library()
+    e = np.zeros(1)
+    assert L.cipsi_ijkl_energy_grad(ctx.handle, 8, 9, V.ctypes.data, h.ctypes.data,
+                                    c.ctypes.data, 8, gamma.ctypes.data, Gam.ctypes.data,
+                                    e.ctypes.data, None) == -1          # n_mo > n_ao
+    assert L.cipsi_ijkl_energy_grad(ctx.handle, 8, 6, V.ctypes.data, h.ctypes.data,
+                                    c.ctypes.data, 8, gamma.ctypes.data, None,
+                                    e.ctypes.data, None) == -1          # null pair density
+
+
+@pytest.fixture(scope="module")
+def cfg4():
+    cfg = synth.CONFIGS[4]
+    n, m = cfg["n_ao"], cfg["n_mo"]
+    B, c, h, gamma, Gam = _inputs(n, m, cfg["seed"], rank=cfg["rank"], decay=cfg["decay"])
+    return dict(n=n, m=m, B=B, c=c, h=h, gamma=gamma, Gam=Gam)
+
+
+def test_grad_central_differences_n160_m100(ctx, cfg4):
+    n, m = cfg4["n"], cfg4["m"]
+    print(f"[adjoint-fd] n={n} m={m}", flush=True)
+    dev = lambda x: torch.as_tensor(np.ascontiguousarray(x), device="cuda")
+    V = synth.eri_packed_torch(cfg4["B"], "cuda")
+    h, gamma, Gam = dev(cfg4["h"]), dev(cfg4["gamma"]), dev(cfg4["Gam"])
+    c = dev(cfg4["c"]).requires_grad_(True)
+    e = TransformEnergy.apply(c, V, h, gamma, Gam, ctx)
+    e.backward()
+    grad = c.grad.detach()
+    assert torch.isfinite(grad).all()
+    step = 1e-6
+    print(f"[adjoint-fd] E={float(e.detach()):.15e} |grad|max={float(grad.abs().max()):.3e}", flush=True)
+    # Directions with O(1) entries, so that every coefficient moves by about the step.
+    # (With a unit-norm direction over all N*M entries the derivative is small and the
+    # central difference is limited by the FP64 spacing of E itself, ulp(E) / 2h.)
+    gen = torch.Generator(device="cpu").manual_seed(4242)
+    dirs = [torch.randn((n, m), generator=gen, dtype=torch.float64).to("cuda") for _ in range(4)]
+    dirs.append(grad / grad.abs().max())
+    with torch.no_grad():
+        for k, d in enumerate(dirs):
+            ep = TransformEnergy.apply(c + step * d, V, h, gamma, Gam, ctx)
+            em = TransformEnergy.apply(c - step * d, V, h, gamma, Gam, ctx)
+            fd = float(ep - em) / (2 * step)
+            an = float(torch.sum(grad * d))
+            print(f"[adjoint-fd] dir {k}: fd={fd:.15e} adjoint={an:.15e} "
+                  f"rel={abs(fd - an) / abs(an):.2e}", flush=True)
+            assert abs(fd - an) <= REL_TOL * abs(an)
+
+
+def test_grad_matches_cpu_fp64_rows_n160_m100(ctx, cfg4):
+    """dE/dC(mu, :) for sampled mu from an FP64 CPU evaluation: the oracle's tpqkld rows
+    H[pair(mu, nu)] for all nu, then 2 sum_{nu,q} C(nu,q) (H Gh)[pair(mu,nu)][pair(p,q)]
+    + 2 (h C gamma)(mu, p), Gh[b][a] = m(b) Gamma(a, b)."""
+    n, m = cfg4["n"], cfg4["m"]
+    print(f"[adjoint-cpu] n={n} m={m}", flush=True)
+    V = synth.eri_packed(cfg4["B"])
+    c, hp, gamma, Gam = cfg4["c"], cfg4["h"], cfg4["gamma"], cfg4["Gam"]
+    e, g = ctx.energy_grad(V, hp, c, gamma, Gam)
+    pm = synth.n_pairs(m)
+    pq = np.array([synth.pair0(max(p, q), min(p, q)) for p in range(m) for q in range(m)])
+    mult = np.where(np.repeat(np.arange(m), np.arange(1, m + 1))
+                    == np.concatenate([np.arange(i + 1) for i in range(m)]), 1.0, 2.0)
+    Gfull = synth.unpack_sym(Gam, pm)
+    Gh = mult[:, None] * Gfull                           # Gh[b][a] = m(b) Gamma(a, b)
+    hc_gamma = synth.unpack_sym(hp, n) @ c @ gamma
+    for mu in (0, 77, n - 1):
+        rows = [synth.pair0(max(mu, nu), min(mu, nu)) for nu in range(n)]
+        H = oracle.tpqkld_rows(V, c, rows, nthreads=8)[rows]           # n x P_M
+        L = H @ Gh                                                     # n x P_M
+        Lq = L[:, pq].reshape(n, m, m)                                 # [nu, p, q]
+        ref = 2 * np.einsum("nq,npq->p", c, Lq) + 2 * hc_gamma[mu]
+        assert _rel(g[mu], ref) <= REL_TOL, f"mu={mu}"
