@@ -14,7 +14,15 @@ Also reported:
                 bounded sample of AO pairs and MO pairs of the same workload,
                 extrapolated to one whole transform.
 
+Scan workload (--workload scan, a separate line): the 64-geometry potential-curve scan
+(N_AO uniform in 60..180, N_MO = N_AO - 8, rank 3 N_AO, one seed per geometry), all
+geometries transformed per step by one graph launch of the grouped kernels with
+autotuned tile variants per row class; reports geometries/s, the FP64 MFMA flop rate
+of the whole launch against the matrix peak, and the same scan run as a loop of
+single transforms for comparison.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|split_fp16|both]
+                       [--workload transform|scan]
 """
 from __future__ import annotations
 
@@ -81,6 +89,84 @@ def cpu_baseline(ao_host, c, h_gpu, out_gpu, n, m, nthreads=1, n_pq=1200, n_kl=1
     return est, t_h1 + t_h2, d1, d2, len(pq_list), len(kl_list)
 
 
+def run_scan(args, torch):
+    """One step = the whole 64-geometry scan in one graph launch."""
+    from cipsi_amd import ijkl, synth
+    scan = synth.batch_scan()
+    sizes = [(g["n_ao"], g["n_mo"]) for g in scan]
+    print(f"[bench-scan] generating {len(scan)} geometries", file=sys.stderr, flush=True)
+    aos, cs = [], []
+    for g in scan:
+        B = synth.eri_factors(g["n_ao"], g["rank"], g["decay"], g["seed"])
+        aos.append(synth.eri_packed_torch(B, "cuda"))
+        cs.append(torch.as_tensor(np.ascontiguousarray(
+            synth.orthonormal(g["n_ao"], g["seed"] + 1)[:, :g["n_mo"]]), device="cuda"))
+    outs = [torch.empty(synth.n_pairs(synth.n_pairs(m)), dtype=torch.float64, device="cuda")
+            for _, m in sizes]
+    torch.cuda.synchronize()
+    ctx = ijkl.Context(0)
+    plan = ijkl.BatchPlan(ctx, [n for n, _ in sizes], [m for _, m in sizes])
+    t0 = time.perf_counter()
+    if args.scan_variants:
+        # "v1,v2;v1,v2;..." per row class, in increasing row-class order
+        for k, pair in enumerate(args.scan_variants.split(";")):
+            v1, v2 = (int(x) for x in pair.split(","))
+            plan.set_class_plan(k, v1, v2)
+        classes = plan.classes()
+    else:
+        classes = plan.autotune(aos, cs, outs)
+    t_tune = time.perf_counter() - t0
+    print(f"[bench-scan] autotuned in {t_tune:.1f} s: {classes}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        plan.run(aos, cs, outs, use_graph=True)
+    torch.cuda.synchronize()
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run(aos, cs, outs, use_graph=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    tot, cnt = ctx.read_timing("batch")
+    ctx.enable_timing(False)
+    ev_ms = tot / cnt if cnt else None
+    # the same scan as a loop of single transforms (default tile plan), for comparison
+    dt_loop = None
+    if not args.no_loop:
+        for g in range(len(sizes)):
+            ctx.transform(aos[g], cs[g])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for g in range(len(sizes)):
+            ctx.transform(aos[g], cs[g])
+        torch.cuda.synchronize()
+        dt_loop = time.perf_counter() - t0
+    flops = sum(flops_half1(n, m) + flops_half2(n, m) for n, m in sizes)
+    n_out = sum(synth.n_pairs(synth.n_pairs(m)) for _, m in sizes)
+    tf = flops / dt / 1e12
+    result = {
+        "metric": "geometries/s (64-geometry scan, N_AO 60..180, one graph launch)",
+        "value": len(sizes) / dt, "unit": "geometries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded low-rank PSD AO integrals per geometry, orthonormal C)",
+        "config": {"workload": "potential-curve scan: 64 geometries, N_AO ~ U{60..180}, "
+                               "N_MO = N_AO - 8, rank 3 N_AO, decay 10",
+                   "n_ao": [n for n, _ in sizes], "mo_integrals_per_step": n_out,
+                   "classes": [{"row_blocks": k[0], "geometries": k[1], "variant_half1": k[2],
+                                "variant_half2": k[3]} for k in classes]},
+        "mo_integrals_per_s": n_out / dt,
+        "roofline": {"bound": "mfma", "kernel": "whole graph (grouped halves + layout)",
+                     "achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tf / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "graph_event_ms": ev_ms},
+        "single_transform_loop": ({"ms": dt_loop * 1e3, "geometries_per_s": len(sizes) / dt_loop}
+                                  if dt_loop else None),
+        "autotune_s": t_tune,
+    }
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +177,11 @@ def main():
     ap.add_argument("--n-mo", type=int, default=240)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--workload", default="transform", choices=["transform", "scan"])
+    ap.add_argument("--scan-variants", default="",
+                    help='scan: fixed tile variants per row class "v1,v2;v1,v2;..." (no autotune)')
+    ap.add_argument("--no-loop", action="store_true",
+                    help="scan: skip the loop-of-single-transforms comparison")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -100,6 +191,10 @@ def main():
 
     import torch
     from cipsi_amd import ijkl
+
+    if args.workload == "scan":
+        run_scan(args, torch)
+        return
 
     n, m = args.n_ao, args.n_mo
     rank, decay, seed = 3 * n, 20.0, 2026

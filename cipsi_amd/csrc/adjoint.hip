@@ -40,26 +40,53 @@ __global__ void k_gamma_hat(const double* __restrict__ gp, int64_t pm, double* _
     }
 }
 
+// Compensated (double-double) accumulation for the energy: error-free two-sum of the
+// running sum with each term, the low parts summed separately (Knuth's TwoSum).
+struct DD {
+    double hi, lo;
+};
+__device__ inline void dd_add(DD& a, double x) {
+#pragma clang fp contract(off)
+    const double s = a.hi + x;
+    const double bp = s - a.hi;
+    const double err = (a.hi - (s - bp)) + (x - bp);
+    a.hi = s;
+    a.lo += err;
+}
+__device__ inline void dd_add(DD& a, const DD& b) {
+#pragma clang fp contract(off)
+    dd_add(a, b.hi);
+    a.lo += b.lo;
+}
+
 // Per-block partial sums of w m(a) m(b) Gamma_ab g_ab over the packed index (w = 2 off
-// the diagonal of the pair supermatrix); a fixed-order final pass makes it deterministic.
+// the diagonal of the pair supermatrix), in double-double; a fixed-order final pass
+// makes the result deterministic.
 constexpr int RB = 256;
+__device__ inline void block_reduce_dd(DD v, DD* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = RB / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) dd_add(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(RB) void k_energy2_partial(const double* __restrict__ g,
                                                         const double* __restrict__ gp,
                                                         int64_t n, double* __restrict__ part) {
-    __shared__ double red[RB];
-    double s = 0.0;
+    __shared__ DD red[RB];
+    DD s{0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
         const int64_t a = pair_row(i), b = i - pair0(a, 0);
         const double w = (a == b) ? 1.0 : 2.0;
-        s += w * pair_mult(a) * pair_mult(b) * gp[i] * g[i];
+        dd_add(s, w * pair_mult(a) * pair_mult(b) * gp[i] * g[i]);
     }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = RB / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
+    block_reduce_dd(s, red);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = red[0].hi;
+        part[2 * blockIdx.x + 1] = red[0].lo;
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 // energy = sum_pq hmo(pq) gamma(p,q) + 1/2 sum(part); one block.
@@ -67,22 +94,19 @@ __global__ __launch_bounds__(RB) void k_energy_final(const double* __restrict__ 
                                                      const double* __restrict__ hmo,
                                                      const double* __restrict__ gamma, int m,
                                                      double* __restrict__ energy) {
-    __shared__ double red[RB];
-    double s = 0.0;
+    __shared__ DD red[RB];
+    DD s{0.0, 0.0};
     for (int i = threadIdx.x; i < m * m; i += RB) {
         const int p = i / m, q = i - p * m;
-        s += hmo[pairsym(p, q)] * gamma[i];
+        dd_add(s, hmo[pairsym(p, q)] * gamma[i]);
     }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = RB / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
-    }
+    block_reduce_dd(s, red);
     if (threadIdx.x == 0) {
-        double e2 = 0.0;
-        for (int i = 0; i < np_; ++i) e2 += part[i];
-        *energy = red[0] + 0.5 * e2;
+        DD e2{0.0, 0.0};
+        for (int i = 0; i < np_; ++i) dd_add(e2, DD{part[2 * i], part[2 * i + 1]});
+        DD e = red[0];
+        dd_add(e, DD{0.5 * e2.hi, 0.5 * e2.lo});
+        *energy = e.hi + e.lo;
     }
 }
 

@@ -77,4 +77,41 @@ hipError_t launch_pad_c(const double* c, int ldc, int n, int m, int np, int mp, 
 hipError_t launch_half_fp64(const HalfArgs& a, int variant, hipStream_t st);
 hipError_t launch_zero(double* p, int64_t n, hipStream_t st);
 
+// Row class (compile-time row-block count) used for a padded AO dimension np.
+int row_class(int np);
+
+// A persistent grouped launch of one half over several transforms whose padded AO
+// dimension falls in one row class.  geoms[g] are the per-transform arguments (device
+// array); item_start (device, ngeom + 1 entries) numbers their work items.
+struct GroupLaunch {
+    const HalfArgs* geoms;
+    const int64_t* item_start;
+    int ngeom;
+    int row_class;
+    int variant;
+    int grid;        // > 0: launch with this grid; <= 0: prepare only
+    int* grid_out;   // prepare: resident-capacity grid (multiple of 8)
+};
+hipError_t launch_half_fp64_group(const GroupLaunch& gl, hipStream_t st);
+
+// Grouped layout kernels over several transforms (one launch each).
+struct LayoutGeom {
+    const double* ao;      // packed AO block
+    const double* c;       // caller's C, column-major, ld ldc
+    double* vsq;           // full AO supermatrix, later H^T
+    double* h;             // H
+    double* cr;            // padded row-major C
+    int64_t pa, pm;
+    int n, m, np, mp, ldc;
+    int64_t expand_start;  // first expand tile of this transform
+    int64_t trans_start;   // first transpose tile
+    int64_t padc_start;    // first pad_c element
+};
+hipError_t launch_group_pad_c(const LayoutGeom* g, int ngeom, int64_t total, hipStream_t st);
+hipError_t launch_group_expand(const LayoutGeom* g, int ngeom, int64_t total_tiles, hipStream_t st);
+hipError_t launch_group_transpose(const LayoutGeom* g, int ngeom, int64_t total_tiles,
+                                  hipStream_t st);
+int64_t expand_tiles(int64_t pa);
+int64_t transpose_tiles(int64_t rows, int64_t cols);
+
 }  // namespace cipsi

@@ -35,6 +35,13 @@ struct HalfCfg {
     static constexpr int EPT_Q = (NRB * 16 * 16 + NT - 1) / NT;   // rows padded to NRB*16
 };
 
+// Global-memory views of the argument pointers: when the arguments come from a device
+// table (grouped launch) the compiler cannot tell that they point to global memory and
+// would emit flat accesses.
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) const int64_t gcint64;
+
 __device__ inline void work_item(const HalfArgs& A, int lt, int64_t item, int64_t& b, int& tile,
                                  int& ncols, int64_t& limit, int64_t& dst_off) {
     if (A.half == 1) {
@@ -48,11 +55,11 @@ __device__ inline void work_item(const HalfArgs& A, int lt, int64_t item, int64_
         int lo = 0, hi = A.m;                      // tile_start[0] = 0 <= item < tile_start[m]
         while (hi - lo > 1) {
             int mid = (lo + hi) >> 1;
-            if (A.tile_start[mid] <= item) lo = mid; else hi = mid;
+            if (((gcint64*)A.tile_start)[mid] <= item) lo = mid; else hi = mid;
         }
         const int l = lo;
         const int nt = (l + lt) / lt;              // ceil((l+1)/lt)
-        const int64_t rem = item - A.tile_start[l];
+        const int64_t rem = item - ((gcint64*)A.tile_start)[l];
         b = pair0(l, 0) + rem / nt;
         tile = (int)(rem % nt);
         ncols = l + 1;
@@ -61,8 +68,10 @@ __device__ inline void work_item(const HalfArgs& A, int lt, int64_t item, int64_
     }
 }
 
-template <int NW, int NRB, int KC, int MINB>
-__global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
+// One work item (matrix b, column tile) of one half.  Every thread of the work-group
+// calls it with the same item; it ends with a barrier, so LDS is free on return.
+template <int NW, int NRB, int KC>
+__device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     using K = HalfCfg<NW, NRB, KC>;
     extern __shared__ double smem[];
     const int tid = threadIdx.x;
@@ -72,11 +81,6 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
     constexpr int RP = NRB * 16;                   // rows held in LDS (zero beyond n)
     constexpr int LDZ = RP + 2;
 
-    // XCD-aware mapping: consecutive work items (same matrix, neighbouring tiles)
-    // land on the same XCD so the Z_b row is fetched into one L2.
-    const int64_t per_xcd = gridDim.x >> 3;
-    const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (item >= A.ntile_total) return;
     int64_t b, limit, dst_off;
     int tile, ncols;
     work_item(A, K::LT, item, b, tile, ncols, limit, dst_off);
@@ -84,8 +88,8 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
     const int c1 = min(c0 + K::LT, ncols);
     const int cw = c0 + 16 * w;                    // first column of this wave
     const bool wave_active = cw < c1;
-    const double* __restrict__ zsrc = A.src + b * A.src_ld;
-    const double* __restrict__ cr = A.cr;
+    gcdouble* __restrict__ zsrc = (gcdouble*)A.src + b * A.src_ld;
+    gcdouble* __restrict__ cr = (gcdouble*)A.cr;
     const int ldm = A.ldm;
     const int nrb = np >> 4;
 
@@ -200,7 +204,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
     };
     const int nab = (c1 - 1) / 16 + 1;
     const int my_last_ab = wave_active ? min(nab - 1, cw / 16) : -1;
-    double* __restrict__ dst = A.dst + dst_off;
+    gdouble* __restrict__ dst = (gdouble*)A.dst + dst_off;
     fetch_q(0);
     commit_q(0);
     __syncthreads();
@@ -236,12 +240,51 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
 }
 
 template <int NW, int NRB, int KC, int MINB>
-static hipError_t launch_v(const HalfArgs& a, hipStream_t st) {
+__global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
+    // XCD-aware mapping: consecutive work items (same matrix, neighbouring tiles)
+    // land on the same XCD so the Z_b row is fetched into one L2.
+    const int64_t per_xcd = gridDim.x >> 3;
+    const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (item >= A.ntile_total) return;
+    half_tile<NW, NRB, KC>(A, item);
+}
+
+// Persistent grouped launch over the work items of several transforms (a scan of
+// geometries).  Items are numbered geometry after geometry (item_start[g] is the
+// first item of geometry g, item_start[ngeom] the total); the item range is cut
+// into 8 contiguous pieces, one per XCD (work-group w runs on XCD w % 8), and the
+// work-groups of an XCD stride through its piece, so neighbouring tiles of one
+// matrix share an L2.  The grid is a multiple of 8 and sized to the resident
+// capacity of the device; every work-group leaves when its piece is done.
+template <int NW, int NRB, int KC, int MINB>
+__global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64_group(const HalfArgs* __restrict__ geoms,
+                                                                  const int64_t* __restrict__ item_start,
+                                                                  int ngeom) {
+    const int64_t total = item_start[ngeom];
+    const int x = blockIdx.x & 7;
+    const int64_t lo = total * x / 8, hi = total * (x + 1) / 8;
+    const int64_t nx = gridDim.x >> 3;
+    int g = 0;
+    for (int64_t it = lo + (blockIdx.x >> 3); it < hi; it += nx) {
+        while (item_start[g + 1] <= it) ++g;       // items only increase: walk forward
+        const HalfArgs A = geoms[g];
+        half_tile<NW, NRB, KC>(A, it - item_start[g]);
+    }
+}
+
+template <int NW, int NRB, int KC>
+static size_t half_shmem() {
     using K = HalfCfg<NW, NRB, KC>;
     const int ldz = NRB * 16 + 2;
     const size_t q1 = 2 * (size_t)(KC * ldz + KC * K::LDC);
     const size_t q2 = 2 * (size_t)NRB * 16 * K::LDQ;
-    const size_t shmem = sizeof(double) * (q1 > q2 ? q1 : q2);
+    return sizeof(double) * (q1 > q2 ? q1 : q2);
+}
+
+template <int NW, int NRB, int KC, int MINB>
+static hipError_t launch_v(const HalfArgs& a, hipStream_t st) {
+    using K = HalfCfg<NW, NRB, KC>;
+    const size_t shmem = half_shmem<NW, NRB, KC>();
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void*)k_half_fp64<NW, NRB, KC, MINB>,
@@ -252,6 +295,34 @@ static hipError_t launch_v(const HalfArgs& a, hipStream_t st) {
     int64_t g = (a.ntile_total + 7) / 8 * 8;
     if (g > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_half_fp64<NW, NRB, KC, MINB>), dim3((unsigned)g), dim3(K::NT), shmem, st, a);
+    return hipGetLastError();
+}
+
+// Grouped launch.  grid <= 0 only prepares the kernel (LDS attribute) and returns in
+// *grid_out the resident-capacity grid (a multiple of 8); grid > 0 launches.
+template <int NW, int NRB, int KC, int MINB>
+static hipError_t launch_group_v(const GroupLaunch& gl, hipStream_t st) {
+    using K = HalfCfg<NW, NRB, KC>;
+    const void* fn = (const void*)k_half_fp64_group<NW, NRB, KC, MINB>;
+    const size_t shmem = half_shmem<NW, NRB, KC>();
+    if (gl.grid <= 0) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024);
+        if (e != hipSuccess) return e;
+        int per_cu = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, K::NT, shmem);
+        if (e != hipSuccess) return e;
+        if (per_cu < 1) per_cu = 1;
+        int dev = 0, ncu = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) !=
+            hipSuccess)
+            return e;
+        if (gl.grid_out) *gl.grid_out = (ncu * per_cu + 7) / 8 * 8;
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL((k_half_fp64_group<NW, NRB, KC, MINB>), dim3((unsigned)gl.grid), dim3(K::NT),
+                       shmem, st, gl.geoms, gl.item_start, gl.ngeom);
     return hipGetLastError();
 }
 
@@ -276,14 +347,49 @@ static hipError_t launch_rows(const HalfArgs& a, int v, hipStream_t st) {
     }
 }
 
+template <int NRB>
+static hipError_t launch_group_rows(const GroupLaunch& gl, hipStream_t st) {
+    switch (gl.variant) {
+        case 0: return launch_group_v<4, NRB, 8, 1>(gl, st);
+        case 1: return launch_group_v<4, NRB, 16, 1>(gl, st);
+        case 2: return launch_group_v<8, NRB, 8, 1>(gl, st);
+        case 3: return launch_group_v<8, NRB, 16, 1>(gl, st);
+        case 4: return launch_group_v<2, NRB, 16, 1>(gl, st);
+        case 5: return launch_group_v<4, NRB, 8, 2>(gl, st);
+        case 6: return launch_group_v<2, NRB, 8, 2>(gl, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+int row_class(int np) {
+    const int nrb = np / 16;
+    if (nrb <= 4) return 4;
+    if (nrb <= 8) return 8;
+    if (nrb <= 12) return 12;
+    if (nrb <= 15) return 15;
+    return 16;
+}
+
 hipError_t launch_half_fp64(const HalfArgs& a, int variant, hipStream_t st) {
     if (a.np > 256 || (a.np & 15) || (a.mp & 15)) return hipErrorInvalidValue;
-    const int nrb = a.np / 16;
-    if (nrb <= 4) return launch_rows<4>(a, variant, st);
-    if (nrb <= 8) return launch_rows<8>(a, variant, st);
-    if (nrb <= 12) return launch_rows<12>(a, variant, st);
-    if (nrb <= 15) return launch_rows<15>(a, variant, st);
-    return launch_rows<16>(a, variant, st);
+    switch (row_class(a.np)) {
+        case 4: return launch_rows<4>(a, variant, st);
+        case 8: return launch_rows<8>(a, variant, st);
+        case 12: return launch_rows<12>(a, variant, st);
+        case 15: return launch_rows<15>(a, variant, st);
+        default: return launch_rows<16>(a, variant, st);
+    }
+}
+
+hipError_t launch_half_fp64_group(const GroupLaunch& gl, hipStream_t st) {
+    switch (gl.row_class) {
+        case 4: return launch_group_rows<4>(gl, st);
+        case 8: return launch_group_rows<8>(gl, st);
+        case 12: return launch_group_rows<12>(gl, st);
+        case 15: return launch_group_rows<15>(gl, st);
+        case 16: return launch_group_rows<16>(gl, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace cipsi

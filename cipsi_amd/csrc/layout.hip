@@ -127,10 +127,9 @@ hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t
 // transposed through LDS, both coalesced.
 constexpr int XT = 64;
 
-__global__ __launch_bounds__(256) void k_expand_packed(const double* __restrict__ v, int64_t pa,
-                                                       double* __restrict__ vsq) {
+__device__ __forceinline__ void expand_tile(const double* __restrict__ v, int64_t pa,
+                                            double* __restrict__ vsq, int64_t t) {
     __shared__ double tile[XT][XT + 1];
-    const int64_t t = blockIdx.x;
     const int64_t I = pair_row(t);
     const int64_t J = t - pair0(I, 0);
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 4
@@ -150,19 +149,24 @@ __global__ __launch_bounds__(256) void k_expand_packed(const double* __restrict_
     }
 }
 
+__global__ __launch_bounds__(256) void k_expand_packed(const double* __restrict__ v, int64_t pa,
+                                                       double* __restrict__ vsq) {
+    expand_tile(v, pa, vsq, blockIdx.x);
+}
+
+int64_t expand_tiles(int64_t pa) { return npairs((pa + XT - 1) / XT); }
+
 hipError_t launch_expand_packed(const double* ao_packed, int64_t pa, double* vsq, hipStream_t st) {
-    const int64_t nt = (pa + XT - 1) / XT;
-    const int64_t ntiles = npairs(nt);
+    const int64_t ntiles = expand_tiles(pa);
     hipLaunchKernelGGL(k_expand_packed, dim3((unsigned)ntiles), dim3(256), 0, st, ao_packed, pa, vsq);
     return hipGetLastError();
 }
 
 // ----------------------------------------------------------------------------- transpose
-__global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ src, int64_t rows,
-                                                   int64_t cols, double* __restrict__ dst,
-                                                   int64_t tiles_c) {
+__device__ __forceinline__ void transpose_tile(const double* __restrict__ src, int64_t rows,
+                                               int64_t cols, double* __restrict__ dst,
+                                               int64_t tiles_c, int64_t t) {
     __shared__ double tile[XT][XT + 1];
-    const int64_t t = blockIdx.x;
     const int64_t ti = t / tiles_c, tj = t - ti * tiles_c;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int64_t r0 = ti * XT, c0 = tj * XT;
@@ -177,11 +181,21 @@ __global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ sr
     }
 }
 
+__global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ src, int64_t rows,
+                                                   int64_t cols, double* __restrict__ dst,
+                                                   int64_t tiles_c) {
+    transpose_tile(src, rows, cols, dst, tiles_c, blockIdx.x);
+}
+
+int64_t transpose_tiles(int64_t rows, int64_t cols) {
+    return ((rows + XT - 1) / XT) * ((cols + XT - 1) / XT);
+}
+
 hipError_t launch_transpose(const double* src, int64_t rows, int64_t cols, double* dst,
                             hipStream_t st) {
-    const int64_t tr = (rows + XT - 1) / XT, tc = (cols + XT - 1) / XT;
-    hipLaunchKernelGGL(k_transpose, dim3((unsigned)(tr * tc)), dim3(256), 0, st, src, rows, cols,
-                       dst, tc);
+    const int64_t tc = (cols + XT - 1) / XT;
+    hipLaunchKernelGGL(k_transpose, dim3((unsigned)transpose_tiles(rows, cols)), dim3(256), 0, st,
+                       src, rows, cols, dst, tc);
     return hipGetLastError();
 }
 
@@ -199,6 +213,65 @@ hipError_t launch_pad_c(const double* c, int ldc, int n, int m, int np, int mp, 
     int64_t tot = (int64_t)np * mp;
     hipLaunchKernelGGL(k_pad_c, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, c, ldc, n,
                        m, np, mp, cr);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------- grouped
+// One launch over every transform of a scan: block / element indices are numbered
+// transform after transform; the owning transform is found by binary search over the
+// start offsets held in the (small, device-resident) table.
+template <typename F>
+__device__ __forceinline__ int find_geom(const LayoutGeom* __restrict__ g, int ngeom, int64_t x,
+                                        F start) {
+    int lo = 0, hi = ngeom;                       // start(g[0]) = 0 <= x
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start(g[mid]) <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_group_expand(const LayoutGeom* __restrict__ g, int ngeom) {
+    const int64_t t = blockIdx.x;
+    const int k = find_geom(g, ngeom, t, [](const LayoutGeom& x) { return x.expand_start; });
+    expand_tile(g[k].ao, g[k].pa, g[k].vsq, t - g[k].expand_start);
+}
+
+__global__ __launch_bounds__(256) void k_group_transpose(const LayoutGeom* __restrict__ g, int ngeom) {
+    const int64_t t = blockIdx.x;
+    const int k = find_geom(g, ngeom, t, [](const LayoutGeom& x) { return x.trans_start; });
+    transpose_tile(g[k].h, g[k].pa, g[k].pm, g[k].vsq, (g[k].pm + XT - 1) / XT,
+                   t - g[k].trans_start);
+}
+
+__global__ __launch_bounds__(256) void k_group_pad_c(const LayoutGeom* __restrict__ g, int ngeom,
+                                                     int64_t total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int k = find_geom(g, ngeom, i, [](const LayoutGeom& x) { return x.padc_start; });
+    const LayoutGeom& G = g[k];
+    const int64_t e = i - G.padc_start;
+    const int r = (int)(e / G.mp), a = (int)(e - (int64_t)r * G.mp);
+    G.cr[e] = (r < G.n && a < G.m) ? G.c[r + (int64_t)a * G.ldc] : 0.0;
+}
+
+hipError_t launch_group_expand(const LayoutGeom* g, int ngeom, int64_t total_tiles, hipStream_t st) {
+    if (total_tiles <= 0 || total_tiles > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_group_expand, dim3((unsigned)total_tiles), dim3(256), 0, st, g, ngeom);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_transpose(const LayoutGeom* g, int ngeom, int64_t total_tiles,
+                                  hipStream_t st) {
+    if (total_tiles <= 0 || total_tiles > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_group_transpose, dim3((unsigned)total_tiles), dim3(256), 0, st, g, ngeom);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_pad_c(const LayoutGeom* g, int ngeom, int64_t total, hipStream_t st) {
+    if (total <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_group_pad_c, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, g,
+                       ngeom, total);
     return hipGetLastError();
 }
 

@@ -45,6 +45,9 @@ EXPORTS = [
     "cipsi_ijkl_reset_timing", "cipsi_ijkl_autotune", "cipsi_ijkl_set_plan", "cipsi_ijkl_get_plan",
     "cipsi_ijkl_num_variants", "cipsi_ijkl_variant_info",
     "cipsi_ijkl_energy_grad", "cipsi_ijkl_energy_grad_dev",
+    "cipsi_ijkl_batch_workspace_bytes", "cipsi_ijkl_batch_create", "cipsi_ijkl_batch_destroy",
+    "cipsi_ijkl_batch_autotune", "cipsi_ijkl_batch_run", "cipsi_ijkl_batch_num_classes",
+    "cipsi_ijkl_batch_class_info", "cipsi_ijkl_batch_set_class_plan",
 ]
 
 
@@ -107,6 +110,15 @@ def load_library(path: str = LIB_PATH):
                                          _vp, _vp]
     L.cipsi_ijkl_energy_grad_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp,
                                              _vp, _vp, _vp]
+    L.cipsi_ijkl_batch_workspace_bytes.restype = _c_i64
+    L.cipsi_ijkl_batch_workspace_bytes.argtypes = [_c_int, _ip, _ip]
+    L.cipsi_ijkl_batch_create.argtypes = [_vp, _c_int, _ip, _ip, ctypes.POINTER(_vp)]
+    L.cipsi_ijkl_batch_destroy.argtypes = [_vp, _vp]
+    L.cipsi_ijkl_batch_autotune.argtypes = [_vp, _vp, _vp, _vp, _ip, _vp]
+    L.cipsi_ijkl_batch_run.argtypes = [_vp, _vp, _vp, _vp, _ip, _vp, _c_int]
+    L.cipsi_ijkl_batch_num_classes.argtypes = [_vp]
+    L.cipsi_ijkl_batch_class_info.argtypes = [_vp, _c_int, _ip, _ip, _ip, _ip]
+    L.cipsi_ijkl_batch_set_class_plan.argtypes = [_vp, _vp, _c_int, _c_int, _c_int]
     _lib = L
     return L
 
@@ -344,6 +356,102 @@ class Context:
             self.handle, n, m, _ptr(arrs[0]), _ptr(arrs[1]), _ptr(cm), ldc, _ptr(arrs[2]),
             _ptr(arrs[3]), ctypes.byref(e), _ptr(g) if want_grad else None), "energy_grad")
         return e.value, g
+
+
+class BatchPlan:
+    """A scan of geometries transformed together (cipsi_ijkl_batch_*): one hipGraph
+    launch of grouped persistent kernels per run.  Inputs are lists of float64 CUDA
+    tensors: packed AO blocks, C (N_AO x N_MO) and preallocated packed MO outputs."""
+
+    def __init__(self, ctx: "Context", n_ao, n_mo):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.n = [int(x) for x in n_ao]
+        self.m = [int(x) for x in n_mo]
+        if len(self.n) != len(self.m):
+            raise ValueError("n_ao and n_mo differ in length")
+        na = (ctypes.c_int * len(self.n))(*self.n)
+        nm = (ctypes.c_int * len(self.m))(*self.m)
+        h = _vp()
+        ctx._check(self.lib.cipsi_ijkl_batch_create(ctx.handle, len(self.n), na, nm,
+                                                    ctypes.byref(h)), "batch_create")
+        self.handle = h
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.cipsi_ijkl_batch_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _arrays(self, aos, cs, mos):
+        g = len(self.n)
+        if not (len(aos) == len(cs) == len(mos) == g):
+            raise ValueError("one AO block, C and output per geometry")
+        ao_p = (_vp * g)()
+        c_p = (_vp * g)()
+        mo_p = (_vp * g)()
+        ldc = (ctypes.c_int * g)()
+        keep = []
+        for i in range(g):
+            n, m = self.n[i], self.m[i]
+            if tuple(cs[i].shape) != (n, m):
+                raise ValueError(f"geometry {i}: C must be {n} x {m}")
+            if aos[i].numel() != n_pairs(n_pairs(n)) or mos[i].numel() != n_pairs(n_pairs(m)):
+                raise ValueError(f"geometry {i}: AO block or output of the wrong length")
+            for t in (aos[i], mos[i]):
+                if not _is_torch(t) or not t.is_cuda or str(t.dtype) != "torch.float64" \
+                        or not t.is_contiguous():
+                    raise TypeError("batch inputs must be contiguous float64 CUDA tensors")
+            cm, ld = Context._cmat(cs[i])
+            keep.append(cm)
+            ao_p[i] = aos[i].data_ptr()
+            c_p[i] = cm.data_ptr()
+            mo_p[i] = mos[i].data_ptr()
+            ldc[i] = ld
+        return ao_p, c_p, ldc, mo_p, keep
+
+    def run(self, aos, cs, mos, use_graph=True):
+        ao_p, c_p, ldc, mo_p, keep = self._arrays(aos, cs, mos)
+        self.ctx._use_torch_stream()
+        self.ctx._check(self.lib.cipsi_ijkl_batch_run(self.ctx.handle, self.handle, ao_p, c_p,
+                                                      ldc, mo_p, int(bool(use_graph))),
+                        "batch_run")
+        self._keep = keep          # column-major copies of C stay alive until the next run
+        return mos
+
+    def autotune(self, aos, cs, mos):
+        ao_p, c_p, ldc, mo_p, keep = self._arrays(aos, cs, mos)
+        self.ctx._use_torch_stream()
+        self.ctx._check(self.lib.cipsi_ijkl_batch_autotune(self.ctx.handle, self.handle, ao_p,
+                                                           c_p, ldc, mo_p), "batch_autotune")
+        self._keep = keep
+        return self.classes()
+
+    def classes(self):
+        """[(row blocks, geometries, variant half 1, variant half 2)] per row class."""
+        out = []
+        for k in range(self.lib.cipsi_ijkl_batch_num_classes(self.handle)):
+            v = [_c_int() for _ in range(4)]
+            self.ctx._check(self.lib.cipsi_ijkl_batch_class_info(
+                self.handle, k, *[ctypes.byref(x) for x in v]), "batch_class_info")
+            out.append(tuple(x.value for x in v))
+        return out
+
+    def set_class_plan(self, k, var_half1, var_half2):
+        self.ctx._check(self.lib.cipsi_ijkl_batch_set_class_plan(
+            self.ctx.handle, self.handle, k, var_half1, var_half2), "batch_set_class_plan")
+
+
+def batch_workspace_bytes(n_ao, n_mo):
+    g = len(n_ao)
+    return load_library().cipsi_ijkl_batch_workspace_bytes(
+        g, (ctypes.c_int * g)(*[int(x) for x in n_ao]), (ctypes.c_int * g)(*[int(x) for x in n_mo]))
 
 
 def variants():

@@ -107,14 +107,28 @@ def psd_trace1(n: int, seed: int, rank: int | None = None) -> np.ndarray:
     return a / np.trace(a)
 
 
-def pair_density(n_mo: int, seed: int):
-    """(gamma, Gamma_packed): a unit-trace PSD one-particle density (M x M, row-major)
-    and a seeded symmetric pair density on the MO pair supermatrix, packed like the MO
-    stream (P_M(P_M+1)/2 doubles, entries ~ N(0, 1) / P_M)."""
+def pair_density(n_mo: int, seed: int, rank: int = 32, row_block: int = 2048):
+    """(gamma, Gamma_packed): seeded one- and two-particle densities, both PSD and
+    trace-normalised.  gamma is M x M (row-major, trace 1).  Gamma lives on the MO pair
+    supermatrix (P_M x P_M, Gamma_pqrs = Gamma[pair(p,q), pair(r,s)]) as F F^T for a
+    Gaussian P_M x rank factor F, scaled so that the 4-index trace
+    sum_pq Gamma_pqpq = sum_a m(a) Gamma_aa is 1 (m(a) = 2 for p != q); it is returned
+    packed like the MO stream (P_M(P_M+1)/2 doubles)."""
     pm = n_pairs(n_mo)
     gamma = psd_trace1(n_mo, seed)
     rng = np.random.default_rng(seed + 1)
-    return gamma, rng.standard_normal(n_pairs(pm)) / pm
+    F = rng.standard_normal((pm, min(rank, pm)))
+    a, b = pair_index_arrays(n_mo)
+    mult = np.where(a == b, 1.0, 2.0)
+    scale = 1.0 / float(np.sum(mult * np.einsum("ij,ij->i", F, F)))
+    out = np.empty(n_pairs(pm))
+    for r0 in range(0, pm, row_block):
+        r1 = min(pm, r0 + row_block)
+        blk = (F[r0:r1] @ F[:r1].T) * scale
+        rows, cols = np.tril_indices(r1)
+        sel = rows >= r0
+        out[pair0(r0, 0):pair0(r1 - 1, r1 - 1) + 1] = blk[rows[sel] - r0, cols[sel]]
+    return gamma, out
 
 
 def packed_lower(a: np.ndarray) -> np.ndarray:

@@ -164,6 +164,41 @@ int cipsi_ijkl_read_timing(cipsi_ijkl_ctx *ctx, const char *stage, double *total
                            int64_t *launches);
 int cipsi_ijkl_reset_timing(cipsi_ijkl_ctx *ctx);
 
+/* ---- Batched scan of geometries --------------------------------------------------
+ * The reference transforms one geometry per ijkl process; its scan driver starts one
+ * process per geometry (Autocip_13/autocip13.f90:525-567, each reaching ijkl.f:318).
+ * A batch plan transforms all geometries of a scan, each with its own N_AO, N_MO, AO
+ * block and C, in one hipGraph launch of grouped persistent kernels; the per-geometry
+ * results are those of cipsi_ijkl_transform_packed_dev.
+ *   create      sizes only; allocates the device workspace of every geometry
+ *               (cipsi_ijkl_batch_workspace_bytes) and per-geometry argument tables.
+ *   autotune    times every tile variant of both halves for each row class of the
+ *               plan on the given data and keeps the fastest (leaves the outputs of a
+ *               full run in d_mo).
+ *   run         d_ao[g], d_c[g] (ldc[g] >= n_ao[g]) and d_mo[g] are device pointers,
+ *               held in host arrays of n_geom entries; asynchronous on the context
+ *               stream.  use_graph != 0 launches the captured graph (captured on the
+ *               first such run and after a plan change), 0 launches the same kernels
+ *               directly.  Runs of one plan must stay on one stream.
+ *   class_info  row class k of the plan: row blocks (padded N_AO / 16 bound), number
+ *               of geometries and the tile variants of the two halves. */
+typedef struct cipsi_ijkl_batch cipsi_ijkl_batch;
+int64_t cipsi_ijkl_batch_workspace_bytes(int n_geom, const int *n_ao, const int *n_mo);
+int cipsi_ijkl_batch_create(cipsi_ijkl_ctx *ctx, int n_geom, const int *n_ao, const int *n_mo,
+                            cipsi_ijkl_batch **plan);
+int cipsi_ijkl_batch_destroy(cipsi_ijkl_ctx *ctx, cipsi_ijkl_batch *plan);
+int cipsi_ijkl_batch_autotune(cipsi_ijkl_ctx *ctx, cipsi_ijkl_batch *plan,
+                              const double *const *d_ao, const double *const *d_c,
+                              const int *ldc, double *const *d_mo);
+int cipsi_ijkl_batch_run(cipsi_ijkl_ctx *ctx, cipsi_ijkl_batch *plan, const double *const *d_ao,
+                         const double *const *d_c, const int *ldc, double *const *d_mo,
+                         int use_graph);
+int cipsi_ijkl_batch_num_classes(const cipsi_ijkl_batch *plan);
+int cipsi_ijkl_batch_class_info(const cipsi_ijkl_batch *plan, int k, int *row_blocks,
+                                int *n_geom, int *var_half1, int *var_half2);
+int cipsi_ijkl_batch_set_class_plan(cipsi_ijkl_ctx *ctx, cipsi_ijkl_batch *plan, int k,
+                                    int var_half1, int var_half2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,12 @@ def test_energy_grad_matches_oracle(ctx, n, m):
     # energy only
     e_only, none = ctx.energy_grad(V, h, c, gamma, Gam, want_grad=False)
     assert none is None and e_only == e
+    # two-electron part alone (h = 0, gamma = 0)
+    z_h, z_g = np.zeros_like(h), np.zeros_like(gamma)
+    e2_ref, g2_ref = oracle.energy_grad(V, z_h, c, z_g, Gam)
+    e2, g2 = ctx.energy_grad(V, z_h, c, z_g, Gam)
+    assert abs(e2 - e2_ref) <= 1e-12 * max(1.0, abs(e2_ref))
+    assert _rel(g2, g2_ref) <= 1e-11
 
 
 def test_energy_grad_rejects_bad_input(ctx):

@@ -1,0 +1,170 @@
+"""Batched scan (cipsi_ijkl_batch_*): every geometry of a scan, transformed by the grouped
+persistent kernels in one graph launch, equals the single-transform path on the same
+inputs bit for bit (the tile variants and row classes only change which work-group
+computes an element, not the order of its sums), and the single path is pinned to the
+oracle and the closed form by tests/test_gpu_parity.py and tests/test_gpu_fullsize.py.
+Here the smallest geometry is also checked against the oracle directly, and one
+geometry of the 64-geometry scan against the closed form.
+
+Graph replays: the captured graph is replayed with new inputs and with eager torch
+work and synchronisations between the replays; every replay is checked."""
+import numpy as np
+import pytest
+
+import oracle
+from cipsi_amd import synth
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from cipsi_amd import ijkl
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = ijkl.Context(0)
+    yield c
+    c.close()
+
+
+def make_geoms(sizes, seed0, rank_factor=2, decay=6.0):
+    aos, cs, Bs = [], [], []
+    for i, (n, m) in enumerate(sizes):
+        B = synth.eri_factors(n, rank_factor * n, decay, seed0 + i)
+        Bs.append(B)
+        aos.append(synth.eri_packed_torch(B, "cuda"))
+        cs.append(torch.as_tensor(np.ascontiguousarray(synth.orthonormal(n, seed0 + 100 + i)[:, :m]),
+                                  device="cuda"))
+    return aos, cs, Bs
+
+
+def outputs(sizes):
+    return [torch.full((synth.n_pairs(synth.n_pairs(m)),), float("nan"), dtype=torch.float64,
+                       device="cuda") for _, m in sizes]
+
+
+def single(ctx, aos, cs):
+    return [ctx.transform(a, c) for a, c in zip(aos, cs)]
+
+
+MIXED = [(20, 12), (70, 62), (33, 33), (100, 92), (65, 57), (130, 122), (16, 5), (140, 140)]
+
+
+def test_batch_matches_single_path_bitwise(ctx):
+    from cipsi_amd import ijkl
+    print(f"[batch-mixed] sizes={MIXED}", flush=True)
+    aos, cs, _ = make_geoms(MIXED, 500)
+    ref = single(ctx, aos, cs)
+    plan = ijkl.BatchPlan(ctx, [n for n, _ in MIXED], [m for _, m in MIXED])
+    classes = plan.classes()
+    assert [k[0] for k in classes] == [4, 8, 12]
+    assert sum(k[1] for k in classes) == len(MIXED)
+    for use_graph in (False, True):
+        out = outputs(MIXED)
+        plan.run(aos, cs, out, use_graph=use_graph)
+        torch.cuda.synchronize()
+        for g, (o, r) in enumerate(zip(out, ref)):
+            assert torch.equal(o, r), f"geometry {g} {MIXED[g]} graph={use_graph}"
+    # the smallest geometries against the oracle as well
+    for g in (0, 6):
+        n, m = MIXED[g]
+        exp = oracle.ijkl(aos[g].cpu().numpy(), cs[g].cpu().numpy())
+        assert np.max(np.abs(out[g].cpu().numpy() - exp)) <= 1e-12
+    plan.close()
+
+
+def test_batch_autotune_and_class_plans(ctx):
+    from cipsi_amd import ijkl
+    sizes = MIXED[:6]
+    aos, cs, _ = make_geoms(sizes, 600)
+    ref = single(ctx, aos, cs)
+    plan = ijkl.BatchPlan(ctx, [n for n, _ in sizes], [m for _, m in sizes])
+    out = outputs(sizes)
+    classes = plan.autotune(aos, cs, out)
+    torch.cuda.synchronize()
+    nv = len(ijkl.variants())
+    for rb, ng, v1, v2 in classes:
+        assert 0 <= v1 < nv and 0 <= v2 < nv and ng >= 1
+    for g in range(len(sizes)):
+        assert torch.equal(out[g], ref[g]), f"autotune output, geometry {g}"
+    # every explicit class plan gives the same bits
+    for v in range(nv):
+        for k in range(len(classes)):
+            plan.set_class_plan(k, v, (v + 3) % nv)
+        out = outputs(sizes)
+        plan.run(aos, cs, out, use_graph=True)
+        torch.cuda.synchronize()
+        for g in range(len(sizes)):
+            assert torch.equal(out[g], ref[g]), f"variant {v}, geometry {g}"
+    plan.close()
+
+
+def test_graph_replays_with_new_inputs(ctx):
+    from cipsi_amd import ijkl
+    sizes = [(24, 20), (72, 64), (40, 32)]
+    plan = ijkl.BatchPlan(ctx, [n for n, _ in sizes], [m for _, m in sizes])
+    for rep in range(4):
+        print(f"[batch-replay] {rep}", flush=True)
+        aos, cs, _ = make_geoms(sizes, 700 + 10 * rep)
+        out = outputs(sizes)
+        plan.run(aos, cs, out, use_graph=True)
+        junk = torch.randn(1 << 20, device="cuda", dtype=torch.float64).sum()   # eager work
+        torch.cuda.synchronize()
+        ref = single(ctx, aos, cs)
+        assert torch.isfinite(junk)
+        for g in range(len(sizes)):
+            assert torch.equal(out[g], ref[g]), f"replay {rep}, geometry {g}"
+    plan.close()
+
+
+def test_batch_rejects_bad_input(ctx):
+    from cipsi_amd import ijkl
+    with pytest.raises(ijkl.IjklError):
+        ijkl.BatchPlan(ctx, [10, 20], [11, 5])          # n_mo > n_ao
+    with pytest.raises(ijkl.IjklError):
+        ijkl.BatchPlan(ctx, [300], [10])                # beyond the compiled row classes
+    assert ijkl.batch_workspace_bytes([10], [12]) == -1
+    plan = ijkl.BatchPlan(ctx, [10, 12], [8, 12])
+    aos, cs, _ = make_geoms([(10, 8), (12, 12)], 800)
+    with pytest.raises(ValueError):
+        plan.run(aos[:1], cs[:1], outputs([(10, 8)]))
+    plan.close()
+
+
+def test_scan_64_geometries(ctx):
+    """The 64-geometry scan (N_AO uniform in 60..180, N_MO = N_AO - 8, rank 3 N_AO):
+    autotuned, one graph launch; sampled geometries equal the single path bit for bit
+    and the smallest one matches the closed form (pq|rs) = sum_P Bt^P_pq Bt^P_rs."""
+    from cipsi_amd import ijkl
+    scan = synth.batch_scan()
+    sizes = [(g["n_ao"], g["n_mo"]) for g in scan]
+    print(f"[scan64] workspace {ijkl.batch_workspace_bytes(*zip(*sizes)) / 2**30:.1f} GiB",
+          flush=True)
+    aos, cs, Bs = [], [], []
+    for g in scan:
+        B = synth.eri_factors(g["n_ao"], g["rank"], g["decay"], g["seed"])
+        aos.append(synth.eri_packed_torch(B, "cuda"))
+        cs.append(torch.as_tensor(np.ascontiguousarray(
+            synth.orthonormal(g["n_ao"], g["seed"] + 1)[:, :g["n_mo"]]), device="cuda"))
+        Bs.append(B)
+    plan = ijkl.BatchPlan(ctx, [n for n, _ in sizes], [m for _, m in sizes])
+    out = outputs(sizes)
+    print(f"[scan64] classes before tuning {plan.classes()}", flush=True)
+    plan.autotune(aos, cs, out)
+    print(f"[scan64] classes after tuning {plan.classes()}", flush=True)
+    out = outputs(sizes)
+    plan.run(aos, cs, out, use_graph=True)
+    torch.cuda.synchronize()
+    for o in out:
+        assert bool(torch.isfinite(o).all())
+    sample = sorted({0, 17, 42, 63, int(np.argmin([n for n, _ in sizes])),
+                     int(np.argmax([n for n, _ in sizes]))})
+    for g in sample:
+        r = ctx.transform(aos[g], cs[g])
+        assert torch.equal(out[g], r), f"geometry {g} {sizes[g]}"
+    g = int(np.argmin([n for n, _ in sizes]))
+    c = cs[g].cpu().numpy()
+    cf = synth.mo_reference_closed_form(Bs[g], c)
+    assert np.max(np.abs(out[g].cpu().numpy() - cf)) <= 1e-11
+    plan.close()

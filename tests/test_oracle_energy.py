@@ -61,6 +61,15 @@ def test_grad_matches_dense_einsum():
     assert np.max(np.abs(g - g_dense)) <= 1e-12 * np.max(np.abs(g_dense))
 
 
+def test_two_electron_grad_matches_dense_einsum():
+    V, h, c, gamma, Gam = _case(7, 6, 15)
+    h, gamma = np.zeros_like(h), np.zeros_like(gamma)
+    e, g = oracle.energy_grad(V, h, c, gamma, Gam)
+    e_dense, g_dense = _dense_energy_grad(V, h, c, gamma, Gam)
+    assert abs(e - e_dense) <= 1e-12 * max(1.0, abs(e_dense))
+    assert np.max(np.abs(g - g_dense)) <= 1e-12 * np.max(np.abs(g_dense))
+
+
 def test_grad_matches_central_differences():
     V, h, c, gamma, Gam = _case(6, 4, 13)
     _, g = oracle.energy_grad(V, h, c, gamma, Gam)
