@@ -73,6 +73,8 @@ static inline int round16(int x) { return (x + 15) & ~15; }
 // Half-2 work table: tile_start[l] = sum_{l'<l} (l'+1) * ceil((l'+1)/lt).
 int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int64_t* total) {
     if (ctx->ts_m != m || ctx->ts_lt != lt) {
+        // the table may still be read by kernels queued on the context stream
+        if (ctx->ts_m != -1) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         std::vector<int64_t> t(m + 1);
         t[0] = 0;
         for (int l = 0; l < m; ++l) t[l + 1] = t[l] + (int64_t)(l + 1) * ((l + lt) / lt);

@@ -125,9 +125,8 @@ def pair_density(n_mo: int, seed: int, rank: int = 32, row_block: int = 2048):
     for r0 in range(0, pm, row_block):
         r1 = min(pm, r0 + row_block)
         blk = (F[r0:r1] @ F[:r1].T) * scale
-        rows, cols = np.tril_indices(r1)
-        sel = rows >= r0
-        out[pair0(r0, 0):pair0(r1 - 1, r1 - 1) + 1] = blk[rows[sel] - r0, cols[sel]]
+        keep = np.arange(r1)[None, :] <= np.arange(r0, r1)[:, None]
+        out[pair0(r0, 0):pair0(r1 - 1, r1 - 1) + 1] = blk[keep]
     return gamma, out
 
 
