@@ -123,7 +123,7 @@ void fill_tables(cipsi_ijkl_batch* b, const double* const* ao, const double* con
         L.padc_start = padc;
         L.expand_start = expd;
         L.trans_start = trans;
-        padc += (int64_t)L.np * L.mp;
+        padc += (int64_t)kCRows * L.mp;
         expd += expand_tiles(L.pa);
         trans += transpose_tiles(L.pa, L.pm);
     }
@@ -307,7 +307,7 @@ int64_t cipsi_ijkl_batch_workspace_bytes(int n_geom, const int* n_ao, const int*
     for (int g = 0; g < n_geom; ++g) {
         const int64_t pa = npairs(n_ao[g]), pm = npairs(n_mo[g]);
         t += align256(8 * pa * pa) + align256(8 * pa * pm) +
-             align256(8 * (size_t)round16(n_ao[g]) * round16(n_mo[g]));
+             align256(8 * (size_t)kCRows * round16(n_mo[g]));
     }
     return t;
 }
@@ -364,7 +364,7 @@ int cipsi_ijkl_batch_create(cipsi_ijkl_ctx* ctx, int n_geom, const int* n_ao, co
         b->off_h.push_back(off);
         off += align256(8 * pa * pm);
         b->off_cr.push_back(off);
-        off += align256(8 * (size_t)round16(b->n[g]) * round16(b->m[g]));
+        off += align256(8 * (size_t)kCRows * round16(b->m[g]));
     }
     b->work_bytes = off;
     // tables

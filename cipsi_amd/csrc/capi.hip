@@ -100,10 +100,11 @@ int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half, int n, int m, int prec
 // --------------------------------------------------------------------------- pipeline pieces
 int run_pad_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_c, int ldc, double** cr) {
     const int np = round16(n), mp = round16(m);
-    int rc = ctx_alloc(ctx, ctx->cr, sizeof(double) * (size_t)np * mp);
+    (void)np;
+    int rc = ctx_alloc(ctx, ctx->cr, sizeof(double) * (size_t)kCRows * mp);
     if (rc) return rc;
     StageTimer tm(ctx, "pad_c");
-    HIP_TRY(ctx, launch_pad_c(d_c, ldc, n, m, np, mp, (double*)ctx->cr.ptr, ctx->stream));
+    HIP_TRY(ctx, launch_pad_c(d_c, ldc, n, m, kCRows, mp, (double*)ctx->cr.ptr, ctx->stream));
     *cr = (double*)ctx->cr.ptr;
     return CIPSI_OK;
 }
@@ -272,7 +273,7 @@ int cipsi_ijkl_synchronize(cipsi_ijkl_ctx* ctx) {
 int64_t cipsi_ijkl_workspace_bytes(int n_ao, int n_mo, int precision, int with_labels) {
     if (check_dims(n_ao, n_mo, n_ao)) return -1;
     const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
-    int64_t b = 8 * (pa * pa + pa * pm) + 8 * (int64_t)round16(n_ao) * round16(n_mo);
+    int64_t b = 8 * (pa * pa + pa * pm) + 8 * (int64_t)kCRows * round16(n_mo);
     if (with_labels) b += 12 * npairs(pa);
     if (precision == CIPSI_PREC_SPLIT_FP16) b += split_workspace_bytes(n_ao, n_mo);
     return b;

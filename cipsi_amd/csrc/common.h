@@ -36,11 +36,20 @@ __device__ inline int64_t pair_row(int64_t x) {
 //   limit = P_M, dst_off = pq * P_M.
 // Half 2 (tijkld): b = MO pair kl, src = H^T rows, limit = kl + 1,
 //   dst_off = pair0(kl, 0); only columns c <= l (kl = pair0(l, k)) are needed.
+// Padded copy of C in 16-column blocks: block a / 16 holds all kCRows rows of its 16
+// columns, row-major, so C(r, a) = cr[((a / 16) * kCRows + r) * 16 + a % 16].  Every
+// row class (up to 16 blocks of 16 rows) reads rows < kCRows; rows >= N_AO and
+// columns >= N_MO are zero.
+constexpr int kCRows = 256;
+__host__ __device__ inline int64_t cr_index(int r, int a) {
+    return ((int64_t)(a >> 4) * kCRows + r) * 16 + (a & 15);
+}
+
 struct HalfArgs {
     const double* src;     // rows of packed symmetric matrices
     int64_t src_ld;        // = P_A
-    const double* cr;      // C row-major, zero padded: cr[r * ldm + a], r < np, a < mp
-    int ldm;               // = mp
+    const double* cr;      // C in 16-column blocks, zero padded (cr_index)
+    int ldm;               // = mp (columns of the padded copy)
     double* dst;
     int n;                 // N_AO
     int m;                 // N_MO

@@ -25,14 +25,10 @@ template <int NW, int NRB, int KC>
 struct HalfCfg {
     static constexpr int NT = NW * 64;
     static constexpr int LT = NW * 16;                       // columns per work-group
-    static constexpr int LDC = LT + 2;
     static constexpr int EP = 16 * KC;                       // panel elements per 16-row pass
     static constexpr int PPS = NT >= EP ? NT / EP : 1;       // passes per sweep of the threads
     static constexpr int SUB = NT >= EP ? 1 : EP / NT;       // sweeps per pass
     static constexpr int NU = ((NRB + PPS - 1) / PPS) * SUB; // loads per thread per panel
-    static constexpr int EPT_C = (KC * LT + NT - 1) / NT;    // C panel elements per thread
-    static constexpr int LDQ = 18;                           // Q2 C-block row stride
-    static constexpr int EPT_Q = (NRB * 16 * 16 + NT - 1) / NT;   // rows padded to NRB*16
 };
 
 // Global-memory views of the argument pointers: when the arguments come from a device
@@ -70,6 +66,12 @@ __device__ inline void work_item(const HalfArgs& A, int lt, int64_t item, int64_
 
 // One work item (matrix b, column tile) of one half.  Every thread of the work-group
 // calls it with the same item; it ends with a barrier, so LDS is free on return.
+//
+// Only the Z_b panels go through LDS (they are shared by all waves).  The C operands
+// are read from global memory (C is a few hundred KB and stays in L2): in Q1 one value
+// per lane per 4-deep K step serves all NRB row blocks; in Q2 every wave reads its own
+// C blocks, so Q2 needs no LDS and no barriers and each wave works through its own
+// triangle (column block k does k + 1 steps) at its own pace.
 template <int NW, int NRB, int KC>
 __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     using K = HalfCfg<NW, NRB, KC>;
@@ -86,14 +88,16 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     work_item(A, K::LT, item, b, tile, ncols, limit, dst_off);
     const int c0 = tile * K::LT;
     const int c1 = min(c0 + K::LT, ncols);
-    const int cw = c0 + 16 * w;                    // first column of this wave
+    // Column block of this wave.  Waves w and w + 4 share a SIMD; with 8 waves they
+    // take blocks s and 7 - s, so that every SIMD gets the same share of the Q2
+    // triangle.
+    const int wc = NW == 8 ? (w < 4 ? w : 11 - w) : w;
+    const int cw = c0 + 16 * wc;                   // first column of this wave
     const bool wave_active = cw < c1;
     gcdouble* __restrict__ zsrc = (gcdouble*)A.src + b * A.src_ld;
     gcdouble* __restrict__ cr = (gcdouble*)A.cr;
-    const int ldm = A.ldm;
-    const int nrb = np >> 4;
 
-    constexpr int PANEL = KC * LDZ + KC * K::LDC;   // doubles per Q1 stage buffer
+    constexpr int PANEL = KC * LDZ;                // doubles per Q1 stage buffer
 
     // ---------------------------------------------------------------- Q1 staging
     // Panel = Z(r, s0 + kk) for r < np, kk < KC, taken in 16-row passes.  Below the
@@ -101,8 +105,6 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     // (kk fastest across lanes); above it (r < s0) the entries of one s are contiguous
     // in r (r fastest across lanes).  Branch-free: invalid elements load index 0.
     double zst[K::NU];
-    double cst[K::EPT_C];
-    // load slot u of this thread -> (pass p, element e within the pass)
     auto zslot = [&](int u, int& p, int& e) {
         const int sweep = u / K::SUB, sub = u % K::SUB;
         const int t = tid + sub * K::NT;
@@ -125,13 +127,6 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
             const int idx = ok ? ((hi * (hi + 1)) >> 1) + lo : 0;
             zst[u] = zsrc[idx];            // masked at commit: no wait on the load here
         }
-#pragma unroll
-        for (int t = 0; t < K::EPT_C; ++t) {
-            const int e = tid + t * K::NT;
-            const int kk = e / K::LT, c = e % K::LT;
-            const bool ok = (e < KC * K::LT) && (c0 + c < A.mp) && (s0 + kk < np);
-            cst[t] = cr[ok ? (int64_t)(s0 + kk) * ldm + c0 + c : 0];
-        }
     };
     auto commit = [&](int s0, int boff) {
 #pragma unroll
@@ -141,13 +136,6 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
             zmap(p, e, s0, r, kk);
             const bool ok = (r < n) && (s0 + kk < n);
             if (p < NRB) smem[boff + kk * LDZ + r] = ok ? zst[u] : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < K::EPT_C; ++t) {
-            const int e = tid + t * K::NT;
-            const int kk = e / K::LT, c = e % K::LT;
-            const bool ok = (c0 + c < A.mp) && (s0 + kk < np);
-            if (e < KC * K::LT) smem[boff + KC * LDZ + kk * K::LDC + c] = ok ? cst[t] : 0.0;
         }
     };
 
@@ -165,77 +153,71 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
         if (more) fetch((ch + 1) * KC);
         if (wave_active) {
             const int zo = (ch & 1) * PANEL;
-            const int co = zo + KC * LDZ;
+            const int s0 = ch * KC;
+            // B operand C(s0 + k4 + lg, cw + lm) (C is zero-padded to kCRows rows),
+            // loaded one 4-deep K step ahead
+            gcdouble* __restrict__ cbw = cr + (int64_t)(cw >> 4) * (kCRows * 16);
+            const int boff = (s0 + lg) * 16 + lm;
+            double bcur = cbw[boff];
 #pragma unroll
             for (int k4 = 0; k4 < KC; k4 += 4) {
-                const double bf = smem[co + (k4 + lg) * K::LDC + 16 * w + lm];
+                const double bnext = k4 + 4 < KC ? cbw[boff + (k4 + 4) * 16] : 0.0;
 #pragma unroll
                 for (int rb = 0; rb < NRB; ++rb) {
                     const double af = smem[zo + (k4 + lg) * LDZ + rb * 16 + lm];
-                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, acc[rb], 0, 0, 0);
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bcur, acc[rb], 0, 0, 0);
                 }
+                bcur = bnext;
             }
         }
         __builtin_amdgcn_sched_barrier(0);
         if (more) commit((ch + 1) * KC, ((ch + 1) & 1) * PANEL);
         __syncthreads();
     }
+    if (!wave_active) return;
 
     // ---------------------------------------------------------------- Q2: O^T = T^T C
-    // C column blocks (16 columns a, all rows) staged through LDS and shared by the
-    // waves; wave w needs a <= its last column, so it stops after block cw/16.
-    double qst[K::EPT_Q];
-    auto fetch_q = [&](int ab) {
-#pragma unroll
-        for (int t = 0; t < K::EPT_Q; ++t) {
-            const int e = tid + t * K::NT;
-            const int r = e >> 4, a = e & 15;
-            const bool ok = r < np;
-            qst[t] = cr[ok ? (int64_t)r * ldm + ab * 16 + a : 0];
-        }
-    };
-    constexpr int QBUF = RP * K::LDQ;               // doubles per Q2 stage buffer
-    auto commit_q = [&](int boff) {
-#pragma unroll
-        for (int t = 0; t < K::EPT_Q; ++t) {
-            const int e = tid + t * K::NT;
-            if ((e >> 4) < RP) smem[boff + (e >> 4) * K::LDQ + (e & 15)] = (e >> 4) < np ? qst[t] : 0.0;
-        }
-    };
+    // K step (rb, q) covers rows r = 16 rb + 4 q + {0..3}; lane (lm, lg) holds
+    // T(16 rb + 4 q + lg, cw + lm) in acc[rb][q] (the A operand) and loads
+    // C(16 rb + 4 q + lg, 16 ab + lm) as the B operand.  Wave w needs a <= its last
+    // column, so it stops after block cw / 16.
     const int nab = (c1 - 1) / 16 + 1;
-    const int my_last_ab = wave_active ? min(nab - 1, cw / 16) : -1;
+    const int my_last_ab = min(nab - 1, cw / 16);
     gdouble* __restrict__ dst = (gdouble*)A.dst + dst_off;
-    fetch_q(0);
-    commit_q(0);
-    __syncthreads();
-    for (int ab = 0; ab < nab; ++ab) {
-        const bool more = (ab + 1) < nab;
-        if (more) fetch_q(ab + 1);
-        if (ab <= my_last_ab) {
-            const int qo = (ab & 1) * QBUF + lg * K::LDQ + lm;
-            f64x4 o0 = f64x4{0.0, 0.0, 0.0, 0.0}, o1 = o0;
+    for (int ab = 0; ab <= my_last_ab; ++ab) {
+        f64x4 o0 = f64x4{0.0, 0.0, 0.0, 0.0}, o1 = o0;
+        // C rows of row block rb (zero-padded to kCRows rows), prefetched QD blocks
+        // ahead through a ring of QD + 1
+        gcdouble* __restrict__ cba = cr + (int64_t)ab * (kCRows * 16);
+        const int qoff = lg * 16 + lm;
+        constexpr int QD = 1;
+        double ring[QD + 1][4];
+        auto load_rb = [&](int rb, double* bq) {
 #pragma unroll
-            for (int rb = 0; rb < NRB; ++rb) {
-                const int ro = qo + rb * 16 * K::LDQ;
-                o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][0], smem[ro], o0, 0, 0, 0);
-                o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][1], smem[ro + 4 * K::LDQ], o1, 0, 0, 0);
-                o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][2], smem[ro + 8 * K::LDQ], o0, 0, 0, 0);
-                o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][3], smem[ro + 12 * K::LDQ], o1, 0, 0, 0);
-            }
-            // lane (lm, lg), element q: O(a = 16 ab + lm, c = cw + lg + 4q)
-            const int a = ab * 16 + lm;
+            for (int q = 0; q < 4; ++q) bq[q] = cba[qoff + (16 * rb + 4 * q) * 16];
+        };
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int c = cw + lg + 4 * q;
-                if (c < c1 && a <= c) {
-                    const int64_t pp = pair0(c, a);
-                    if (pp < limit) dst[pp] = o0[q] + o1[q];
-                }
+        for (int rb = 0; rb < QD && rb < NRB; ++rb) load_rb(rb, ring[rb]);
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb) {
+            if (rb + QD < NRB) load_rb(rb + QD, ring[(rb + QD) % (QD + 1)]);
+            const double* bq = ring[rb % (QD + 1)];
+            o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][0], bq[0], o0, 0, 0, 0);
+            o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][1], bq[1], o1, 0, 0, 0);
+            o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][2], bq[2], o0, 0, 0, 0);
+            o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][3], bq[3], o1, 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // lane (lm, lg), element q: O(a = 16 ab + lm, c = cw + lg + 4q)
+        const int a = ab * 16 + lm;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = cw + lg + 4 * q;
+            if (c < c1 && a <= c) {
+                const int64_t pp = pair0(c, a);
+                if (pp < limit) dst[pp] = o0[q] + o1[q];
             }
         }
-        __builtin_amdgcn_sched_barrier(0);
-        if (more) commit_q(((ab + 1) & 1) * QBUF);
-        __syncthreads();
     }
 }
 
@@ -274,11 +256,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64_group(const HalfArg
 
 template <int NW, int NRB, int KC>
 static size_t half_shmem() {
-    using K = HalfCfg<NW, NRB, KC>;
-    const int ldz = NRB * 16 + 2;
-    const size_t q1 = 2 * (size_t)(KC * ldz + KC * K::LDC);
-    const size_t q2 = 2 * (size_t)NRB * 16 * K::LDQ;
-    return sizeof(double) * (q1 > q2 ? q1 : q2);
+    return sizeof(double) * 2 * (size_t)KC * (NRB * 16 + 2);
 }
 
 template <int NW, int NRB, int KC, int MINB>
@@ -329,20 +307,20 @@ static hipError_t launch_group_v(const GroupLaunch& gl, hipStream_t st) {
 // Tile variants: (waves per work-group, Q1 panel depth, work-groups per CU hint).
 // Columns per work-group = 16 x waves.
 const HalfVariant kHalfVariants[] = {
-    {4, 8, 1}, {4, 16, 1}, {8, 8, 1}, {8, 16, 1}, {2, 16, 1}, {4, 8, 2}, {2, 8, 2},
+    {4, 16, 1}, {4, 32, 1}, {8, 16, 1}, {8, 32, 1}, {2, 16, 1}, {4, 16, 2}, {8, 8, 1},
 };
 const int kNumHalfVariants = sizeof(kHalfVariants) / sizeof(kHalfVariants[0]);
 
 template <int NRB>
 static hipError_t launch_rows(const HalfArgs& a, int v, hipStream_t st) {
     switch (v) {
-        case 0: return launch_v<4, NRB, 8, 1>(a, st);
-        case 1: return launch_v<4, NRB, 16, 1>(a, st);
-        case 2: return launch_v<8, NRB, 8, 1>(a, st);
-        case 3: return launch_v<8, NRB, 16, 1>(a, st);
+        case 0: return launch_v<4, NRB, 16, 1>(a, st);
+        case 1: return launch_v<4, NRB, 32, 1>(a, st);
+        case 2: return launch_v<8, NRB, 16, 1>(a, st);
+        case 3: return launch_v<8, NRB, 32, 1>(a, st);
         case 4: return launch_v<2, NRB, 16, 1>(a, st);
-        case 5: return launch_v<4, NRB, 8, 2>(a, st);
-        case 6: return launch_v<2, NRB, 8, 2>(a, st);
+        case 5: return launch_v<4, NRB, 16, 2>(a, st);
+        case 6: return launch_v<8, NRB, 8, 1>(a, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -350,13 +328,13 @@ static hipError_t launch_rows(const HalfArgs& a, int v, hipStream_t st) {
 template <int NRB>
 static hipError_t launch_group_rows(const GroupLaunch& gl, hipStream_t st) {
     switch (gl.variant) {
-        case 0: return launch_group_v<4, NRB, 8, 1>(gl, st);
-        case 1: return launch_group_v<4, NRB, 16, 1>(gl, st);
-        case 2: return launch_group_v<8, NRB, 8, 1>(gl, st);
-        case 3: return launch_group_v<8, NRB, 16, 1>(gl, st);
+        case 0: return launch_group_v<4, NRB, 16, 1>(gl, st);
+        case 1: return launch_group_v<4, NRB, 32, 1>(gl, st);
+        case 2: return launch_group_v<8, NRB, 16, 1>(gl, st);
+        case 3: return launch_group_v<8, NRB, 32, 1>(gl, st);
         case 4: return launch_group_v<2, NRB, 16, 1>(gl, st);
-        case 5: return launch_group_v<4, NRB, 8, 2>(gl, st);
-        case 6: return launch_group_v<2, NRB, 8, 2>(gl, st);
+        case 5: return launch_group_v<4, NRB, 16, 2>(gl, st);
+        case 6: return launch_group_v<8, NRB, 8, 1>(gl, st);
         default: return hipErrorInvalidValue;
     }
 }

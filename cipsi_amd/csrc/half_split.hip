@@ -76,14 +76,14 @@ __global__ void k_split_cslices(const double* __restrict__ cr, int ldm, int np, 
     if (c >= mp) return;
     int e = INT_MIN;
     for (int r = 0; r < np; ++r) {
-        const double x = cr[(int64_t)r * ldm + c];
+        const double x = cr[cr_index(r, c)];
         if (x != 0.0) e = max(e, exp_above(x));
     }
     if (e == INT_MIN) e = 0;
     cexp[c] = e;
     const int kc = kp / KCH;
     for (int r = 0; r < kp; ++r) {
-        const double x = r < np ? cr[(int64_t)r * ldm + c] : 0.0;
+        const double x = r < np ? cr[cr_index(r, c)] : 0.0;
         _Float16 d[S];
         digits<S>(x, e, d);
         for (int s = 0; s < S; ++s)

@@ -34,7 +34,7 @@ def test_strerror_and_workspace_without_gpu():
     assert b"label" in lib.cipsi_ijkl_strerror(-2)
     # 240 / 240 FP64: symmetric pair matrix + H + padded C
     pa = 240 * 241 // 2
-    assert ijkl.workspace_bytes(240, 240) == 8 * (pa * pa + pa * pa) + 8 * 240 * 240
+    assert ijkl.workspace_bytes(240, 240) == 8 * (pa * pa + pa * pa) + 8 * 256 * 240   # C padded to 256 rows
     assert ijkl.workspace_bytes(10, 11) == -1          # n_mo > n_ao rejected
     assert ijkl.workspace_bytes(300, 10) == -1         # beyond the supported N_AO
 
