@@ -37,7 +37,7 @@ struct cipsi_ijkl_ctx {
     int ts_m = -1, ts_lt = -1;
     int64_t ts_total = 0;
     // tile plans (kernel variant index) for the two halves: defaults and per-size plans
-    int var_half1 = 3;
+    int var_half1 = 2;
     int var_half2 = 5;
     std::map<std::tuple<int, int, int>, std::pair<int, int>> plans;
     // per-stage event timing

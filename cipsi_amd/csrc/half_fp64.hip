@@ -81,7 +81,7 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     const int lm = lane & 15, lg = lane >> 4;
     const int np = A.np, n = A.n;
     constexpr int RP = NRB * 16;                   // rows held in LDS (zero beyond n)
-    constexpr int LDZ = RP + 2;
+    constexpr int LDZ = RP + 1;                    // odd: conflict-free panel commits
 
     int64_t b, limit, dst_off;
     int tile, ncols;
@@ -190,7 +190,7 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
         // ahead through a ring of QD + 1
         gcdouble* __restrict__ cba = cr + (int64_t)ab * (kCRows * 16);
         const int qoff = lg * 16 + lm;
-        constexpr int QD = 1;
+        constexpr int QD = KC == 16 ? 3 : 1;           // registers allow a deeper ring at KC 16
         double ring[QD + 1][4];
         auto load_rb = [&](int rb, double* bq) {
 #pragma unroll
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64_group(const HalfArg
 
 template <int NW, int NRB, int KC>
 static size_t half_shmem() {
-    return sizeof(double) * 2 * (size_t)KC * (NRB * 16 + 2);
+    return sizeof(double) * 2 * (size_t)KC * (NRB * 16 + 1);
 }
 
 template <int NW, int NRB, int KC, int MINB>
