@@ -43,7 +43,7 @@ FP64_PEAK_TFLOPS = 78.6
 # WRITE_SIZE runs of this workload (fetched = 2.0 x FETCH_SIZE, written = 1.0 x
 # WRITE_SIZE, factors from a 1 GiB copy in the same process); keyed by
 # (n_ao, n_mo, precision).  Summaries: profiles/fp64_n240_kernels.md.
-MEASURED_TRAFFIC = {(240, 240, "fp64"): (2.0 * 3924450.4 + 6780965.6) * 1024}
+MEASURED_TRAFFIC = {(240, 240, "fp64"): (2.0 * 3782600.0 + 6786500.0) * 1024}
 FP16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 
@@ -64,6 +64,55 @@ def make_inputs(n, m, rank, decay, seed, torch):
     ao = synth.eri_packed_torch(B, "cuda")
     c = synth.orthonormal(n, seed + 1)[:, :m]
     return B, ao, torch.as_tensor(np.ascontiguousarray(c), device="cuda"), c
+
+
+def time_transform(ctx, ao, c_dev, prec, args, torch):
+    """Warm up, then time args.steps whole transforms; per-stage HIP-event means."""
+    kw = {"precision": prec}
+    if prec == "split_fp16":
+        kw["n_slices"] = args.slices
+    for _ in range(args.warmup):
+        out = ctx.transform(ao, c_dev, **kw)
+    torch.cuda.synchronize()
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = ctx.transform(ao, c_dev, **kw)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.enable_timing(False)
+    stages = {}
+    for st in ("pad_c", "expand", "half1", "transpose", "half2"):
+        tot, cnt = ctx.read_timing(st)
+        stages[st] = tot / cnt if cnt else 0.0
+    return {"out": out, "ms_step": 1e3 * dt / args.steps, "stages": stages,
+            "slices": (args.slices or 6) if prec == "split_fp16" else 0}
+
+
+def split_flops_executed(n, m, s):
+    """FP16 MFMA flops the split kernels execute (tiles of 32 rows x 32-column wave
+    blocks, K padded to 32; digit products s(s+1)/2)."""
+    kp = (n + 31) // 32 * 32
+    pa, pm = n * (n + 1) // 2, m * (m + 1) // 2
+    prods = s * (s + 1) // 2
+    cols = lambda nc: (nc + 31) // 32 * 32
+    f1 = pa * kp * cols(m) * kp                                  # half 1 step 1
+    f2 = pa * sum(cols(min(32 * (t + 1), m)) for t in range((m + 31) // 32)) * 32 * kp
+    f3 = sum((l + 1) * kp * cols(l + 1) * kp for l in range(m))   # half 2 step 1
+    f4 = sum((l + 1) * sum(cols(min(32 * (t + 1), l + 1)) for t in range((l + 32) // 32)) * 32 * kp
+             for l in range(m))
+    return 2.0 * prods * (f1 + f2 + f3 + f4)
+
+
+def split_roofline(n, m, run, args):
+    fl = split_flops_executed(n, m, run["slices"])
+    t = run["stages"]["half1"] + run["stages"]["half2"]
+    achieved = fl / (t * 1e-3) / 1e12 if t else 0.0
+    return {"bound": "mfma", "kernel": "k_split_gemm (both halves, executed FP16 MFMA flops)",
+            "achieved": round(achieved, 3), "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP16_PEAK_TFLOPS, 4), "traffic": None,
+            "stage_ms": {k: round(v, 4) for k, v in run["stages"].items()}}
 
 
 def cpu_baseline(ao_host, c, h_gpu, out_gpu, n, m, nthreads=1, n_pq=1200, n_kl=1600):
@@ -172,7 +221,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--precision", default="both", choices=["fp64", "split_fp16", "both"])
+    ap.add_argument("--slices", type=int, default=0, help="split-FP16 slice count (0: library default)")
     ap.add_argument("--n-ao", type=int, default=240)
     ap.add_argument("--n-mo", type=int, default=240)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
@@ -214,50 +264,54 @@ def main():
         del src, dst
 
     ctx = ijkl.Context(0)
-    prec = args.precision
-    for _ in range(args.warmup):
-        out = ctx.transform(ao, c_dev, precision=prec)
-    torch.cuda.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
+    precs = ["fp64", "split_fp16"] if args.precision == "both" else [args.precision]
+    runs = {}
+    for prec in precs:
+        r = time_transform(ctx, ao, c_dev, prec, args, torch)
+        runs[prec] = r
+        print(f"[bench] {prec}: {r['ms_step']:.3f} ms/step, stages(ms): "
+              + ", ".join(f"{k}={v:.3f}" for k, v in r["stages"].items()), file=sys.stderr,
+              flush=True)
+    main_prec = precs[0]
+    out = runs[main_prec]["out"]
+    ms_step = runs[main_prec]["ms_step"]
+    value = n_out / (ms_step * 1e-3)
+    stages = runs[main_prec]["stages"]
+    if main_prec == "fp64":
+        f1 = flops_half1(n, m)
+        achieved = f1 / (stages["half1"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "kernel": "k_half_fp64 (half 1: Q1+Q2)",
+                    "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                    "traffic": MEASURED_TRAFFIC.get((n, m, "fp64")),
+                    "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+                    "half2_tflops": round(flops_half2(n, m) / (stages["half2"] * 1e-3) / 1e12, 3)
+                    if stages["half2"] else None}
+    else:
+        roofline = split_roofline(n, m, runs[main_prec], args)
 
-    ctx.enable_timing(True)
-    ctx.reset_timing()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = ctx.transform(ao, c_dev, precision=prec)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ctx.enable_timing(False)
-    ms_step = 1e3 * dt / args.steps
-    value = n_out / (dt / args.steps)
-    stages = {}
-    for st in ("pad_c", "expand", "half1", "transpose", "half2"):
-        tot, cnt = ctx.read_timing(st)
-        stages[st] = tot / cnt if cnt else 0.0
-    print(f"[bench] {ms_step:.3f} ms/step, stages(ms): "
-          + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()), file=sys.stderr, flush=True)
-
-    f1 = flops_half1(n, m)
-    achieved = f1 / (stages["half1"] * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": "k_half_fp64 (half 1: Q1+Q2)",
-                "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                "traffic": MEASURED_TRAFFIC.get((n, m, prec)),
-                "stage_ms": {k: round(v, 4) for k, v in stages.items()},
-                "half2_tflops": round(flops_half2(n, m) / (stages["half2"] * 1e-3) / 1e12, 3)
-                if stages["half2"] else None}
-
-    # accuracy against the closed form on the GPU (independent rounding path)
     result = {
         "metric": METRIC, "value": value, "unit": "MO-integrals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if main_prec == "fp64" else "f16x%d-slices/f32-acc/f64-out" % runs[main_prec]["slices"],
         "data": "synthetic (seeded low-rank PSD AO integrals, orthonormal C)",
         "config": {"workload": f"ijkl C1 transform N_AO={n} N_MO={m} rank={rank} decay={decay}",
-                   "n_ao": n, "n_mo": m, "precision": prec, "mo_integrals_per_step": n_out},
+                   "n_ao": n, "n_mo": m, "precision": main_prec, "mo_integrals_per_step": n_out},
         "roofline": roofline,
     }
+    if "split_fp16" in runs and main_prec == "fp64":
+        sp = runs["split_fp16"]
+        err = float((sp["out"] - out).abs().max())
+        result["split_fp16"] = {
+            "value": n_out / (sp["ms_step"] * 1e-3), "unit": "MO-integrals/s",
+            "ms_per_step": sp["ms_step"], "slices": sp["slices"],
+            "vs_fp64": runs["fp64"]["ms_step"] / sp["ms_step"],
+            "max_abs_err_vs_fp64": err, "tolerance": 1e-10, "within_tolerance": err <= 1e-10,
+            "stage_ms": {k: round(v, 4) for k, v in sp["stages"].items()},
+            "roofline": split_roofline(n, m, sp, args),
+        }
+        print(f"[bench] split_fp16 max abs err vs fp64 {err:.3e}", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         print("[bench] cpu baseline sample", file=sys.stderr, flush=True)
