@@ -8,9 +8,9 @@
 // Every launch is grouped over the geometries: its kernels read the per-geometry
 // arguments from device tables, so the captured graph does not depend on the caller's
 // pointers and is reused by every run of the plan (a run uploads the tables, then
-// launches the graph).  Geometries are grouped by row class (padded N_AO / 16 rounded
-// up to the compiled row-block counts); the autotuner picks the tile variant of each
-// half per class by timing the grouped launches on the plan's own data.
+// launches the graph).  Geometries are grouped by row class (the number of 16-row
+// blocks of N_AO, at least 4); the autotuner picks the tile variant of each half per
+// class by timing the grouped launches on the plan's own data.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -342,9 +342,8 @@ int cipsi_ijkl_batch_create(cipsi_ijkl_ctx* ctx, int n_geom, const int* n_ao, co
     b->n.assign(n_ao, n_ao + n_geom);
     b->m.assign(n_mo, n_mo + n_geom);
     // classes, in increasing row class; geometries keep scan order inside a class
-    const int order[5] = {4, 8, 12, 15, 16};
     int first = 0;
-    for (int rc_ : order) {
+    for (int rc_ = 4; rc_ <= 16; ++rc_) {
         ClassPlan cp;
         cp.row_class = rc_;
         for (int g = 0; g < n_geom; ++g)

@@ -339,33 +339,30 @@ static hipError_t launch_group_rows(const GroupLaunch& gl, hipStream_t st) {
     }
 }
 
+// Compiled row-block counts: every count from 4 to 16 (N_AO up to 256), so a transform
+// computes no padded rows beyond its last 16-row block.
 int row_class(int np) {
     const int nrb = np / 16;
-    if (nrb <= 4) return 4;
-    if (nrb <= 8) return 8;
-    if (nrb <= 12) return 12;
-    if (nrb <= 15) return 15;
-    return 16;
+    return nrb < 4 ? 4 : nrb;
 }
+
+#define CIPSI_ROW_CLASSES(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
 
 hipError_t launch_half_fp64(const HalfArgs& a, int variant, hipStream_t st) {
     if (a.np > 256 || (a.np & 15) || (a.mp & 15)) return hipErrorInvalidValue;
     switch (row_class(a.np)) {
-        case 4: return launch_rows<4>(a, variant, st);
-        case 8: return launch_rows<8>(a, variant, st);
-        case 12: return launch_rows<12>(a, variant, st);
-        case 15: return launch_rows<15>(a, variant, st);
-        default: return launch_rows<16>(a, variant, st);
+#define CIPSI_CASE(R) case R: return launch_rows<R>(a, variant, st);
+        CIPSI_ROW_CLASSES(CIPSI_CASE)
+#undef CIPSI_CASE
+        default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_half_fp64_group(const GroupLaunch& gl, hipStream_t st) {
     switch (gl.row_class) {
-        case 4: return launch_group_rows<4>(gl, st);
-        case 8: return launch_group_rows<8>(gl, st);
-        case 12: return launch_group_rows<12>(gl, st);
-        case 15: return launch_group_rows<15>(gl, st);
-        case 16: return launch_group_rows<16>(gl, st);
+#define CIPSI_CASE(R) case R: return launch_group_rows<R>(gl, st);
+        CIPSI_ROW_CLASSES(CIPSI_CASE)
+#undef CIPSI_CASE
         default: return hipErrorInvalidValue;
     }
 }
