@@ -169,12 +169,19 @@ def ijfil_namelist(prefix: str = "") -> str:
     return "&ijfil prefix='%s' /\n" % prefix if prefix else "&ijfil /\n"
 
 
-def ijklin_namelist(nrec: int, nvecf: int = 0, frozen: int = 0, ydp: bool = True) -> str:
+def ijklin_namelist(nrec: int, nvecf: int = 0, frozen: int = 0, ydp: bool = True,
+                    precision: int | None = None, nslice: int | None = None) -> str:
+    """&ijklin for a C1 run (ijkl.f:28-33); precision / nslice are keys of the GPU host
+    only (0 = FP64 MFMA, 1 = split FP16 with nslice slices)."""
     parts = ["ydp=%s" % ("T" if ydp else "F"), "nrec=%d" % nrec, "yauto=T", "chstan=F"]
     if nvecf:
         parts.append("nvecf=%d" % nvecf)
     if frozen:
         parts.append("ygel(1:%d)=%s" % (frozen, ",".join(["T"] * frozen)))
+    if precision is not None:
+        parts.append("precision=%d" % precision)
+    if nslice is not None:
+        parts.append("nslice=%d" % nslice)
     return "&ijklin " + ", ".join(parts) + " /\n"
 
 

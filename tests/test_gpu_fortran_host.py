@@ -56,3 +56,44 @@ def test_fortran_host_matches_reference_files(name):
         nr = int(g["n_mo_records"])
         assert np.max(np.abs(f50["mo"][(nr - 1) * formats.LSIZE:] - g["last_record"])) <= 1e-12
         assert np.max(np.abs(f50["tail"] - g["tail"])) <= 1e-12
+
+
+def _run_host(g, wd, **keys):
+    n, frozen = int(g["n_ao"]), int(g["frozen"])
+    case = synth.make_case(n, int(g["n_mo"]), int(g["rank"]), float(g["decay"]), int(g["seed"]),
+                           frozen=frozen)
+    xx, ix = formats.labels_from_packed(case["ao_packed"], n, cutoff=float(g["cutoff"]))
+    formats.write_info(os.path.join(wd, "info"), n, g["c_full"], g["h_packed"], enuc=1.25,
+                       n_elec=2)
+    nrec = formats.write_pqrs(os.path.join(wd, "pqrs"), xx, ix)
+    stdin = formats.ijfil_namelist() + formats.ijklin_namelist(nrec, nvecf=n, frozen=frozen,
+                                                               **keys)
+    return subprocess.run([HOST], input=stdin.encode(), cwd=wd, capture_output=True, timeout=120)
+
+
+def test_fortran_host_split_precision():
+    """precision=1, nslice=6 through the namelist: the f50 stream stays within 1e-10 of
+    the reference's (the split path's tolerance); an unbuilt slice count is reported with
+    the Autocip error marker and tag 9 (Autocip_13/calcul.f90:169-176)."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    if not os.path.exists(HOST):
+        pytest.fail("ijkl_gpu is not built")
+    print("[fortran host] split precision", flush=True)
+    g = load_golden("cfg0_n40")
+    n = int(g["n_ao"])
+    with tempfile.TemporaryDirectory() as wd:
+        res = _run_host(g, wd, precision=1, nslice=6)
+        log = res.stdout.decode(errors="replace")
+        assert res.returncode == 0 and "fin ijkl" in log, log[-2000:]
+        f50 = formats.read_f50(os.path.join(wd, "f50"), n)
+    assert np.max(np.abs(f50["mo"][g["sample_idx"]] - g["sample_val"])) <= 1e-10
+    nr = int(g["n_mo_records"])
+    assert np.max(np.abs(f50["mo"][(nr - 1) * formats.LSIZE:] - g["last_record"])) <= 1e-10
+    with tempfile.TemporaryDirectory() as wd:
+        res = _run_host(g, wd, precision=1, nslice=9)
+        log = res.stdout.decode(errors="replace")
+        assert " !! ERROR in ijkl" in log, log[-2000:]
+        after = log[log.index(" !! ERROR in ijkl"):].splitlines()
+        assert "9" in [x.strip() for x in after[1:]], log[-2000:]

@@ -93,3 +93,29 @@ def test_cfg2_n240_fp64_full_stream(ctx):
     o = out.cpu().numpy()
     idx = np.concatenate([np.arange(k * (k + 1) // 2, k * (k + 1) // 2 + k + 1) for k in kl_list])
     assert np.max(np.abs(o_ref[idx] - o[idx])) <= 1e-12
+
+
+@pytest.mark.parametrize("n,m", [(72, 64), (90, 90), (100, 92), (136, 128), (170, 160),
+                                 (200, 190), (216, 216), (256, 256)])
+def test_every_row_class_against_closed_form(ctx, n, m):
+    """One size per compiled row-block count not covered above (5, 6, 7, 9, 11, 13, 14 and
+    the largest, 16 = N_AO 256): the whole packed stream against the closed form, and the
+    first and last MO pairs of the second half against the oracle's tijkld."""
+    print(f"[row class] n={n} m={m} blocks={(n + 15) // 16}", flush=True)
+    B = synth.eri_factors(n, 2 * n, 12.0, 4000 + n)
+    V = synth.eri_packed_torch(B, "cuda")
+    c = np.ascontiguousarray(synth.orthonormal(n, 4100 + n)[:, :m])
+    cd = torch.as_tensor(c, device="cuda")
+    out = ctx.transform(V, cd)
+    ref = closed_form_packed(B, c)
+    diff = float(torch.max(torch.abs(out - ref)))
+    print(f"  max|gpu - closed form| = {diff:.3e}", flush=True)
+    assert diff <= 1e-11
+    del ref
+    h = ctx.tpqkld(V, cd).cpu().numpy()
+    pm = m * (m + 1) // 2
+    kl_list = np.array([0, pm // 2, pm - 1], dtype=np.int64)
+    o_ref = oracle.tijkld_rows(h, c, kl_list)
+    o = out.cpu().numpy()
+    idx = np.concatenate([np.arange(k * (k + 1) // 2, k * (k + 1) // 2 + k + 1) for k in kl_list])
+    assert np.max(np.abs(o_ref[idx] - o[idx])) <= 1e-12
