@@ -138,6 +138,92 @@ def cpu_baseline(ao_host, c, h_gpu, out_gpu, n, m, nthreads=1, n_pq=1200, n_kl=1
     return est, t_h1 + t_h2, d1, d2, len(pq_list), len(kl_list)
 
 
+def labels_on_device(V, n, torch, seed=7, chunk=1 << 26, shuffle=True):
+    """The pshf label stream of the packed block V (canonical labels, values divided by
+    2 for i == j, k == l and ij == kl, as pshf/qout.f:52-77 writes them), built on the
+    device.  In packed order it is the order pshf writes for a basis of s shells only
+    (its shell-quartet loops then run over single functions); shuffled it is the worst
+    case of a scattered stream."""
+    npk = V.numel()
+    xx = torch.empty(npk, dtype=torch.float64, device="cuda")
+    lab = torch.empty(npk, dtype=torch.int32, device="cuda")
+
+    def row(x):
+        r = torch.floor((torch.sqrt(8.0 * x.double() + 1.0) - 1.0) * 0.5).long()
+        r = torch.where(r * (r + 1) // 2 > x, r - 1, r)
+        return torch.where((r + 1) * (r + 2) // 2 <= x, r + 1, r)
+
+    for a0 in range(0, npk, chunk):
+        idx = torch.arange(a0, min(npk, a0 + chunk), device="cuda", dtype=torch.int64)
+        ab = row(idx)
+        cd = idx - ab * (ab + 1) // 2
+        i = row(ab)
+        j = ab - i * (i + 1) // 2
+        k = row(cd)
+        l = cd - k * (k + 1) // 2
+        word = ((i + 1) << 24) | ((j + 1) << 16) | ((k + 1) << 8) | (l + 1)
+        word = torch.where(word >= 2**31, word - 2**32, word)
+        lab[a0:a0 + idx.numel()] = word.to(torch.int32)
+        scale = torch.ones_like(idx, dtype=torch.float64)
+        scale = torch.where(i == j, scale * 0.5, scale)
+        scale = torch.where(k == l, scale * 0.5, scale)
+        scale = torch.where(ab == cd, scale * 0.5, scale)
+        xx[a0:a0 + idx.numel()] = V[a0:a0 + idx.numel()] * scale
+    if not shuffle:
+        return xx, lab
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    perm = torch.randperm(npk, device="cuda", generator=g)
+    return xx[perm], lab[perm]
+
+
+def run_labels(args, torch):
+    """Label ingest (the transd step on the GPU): N_AO=240 label stream -> packed AO
+    block, HBM-bound; reports GB/s of algorithmic traffic (12 B read and 8 B written per
+    label) for the stream in pshf order (packed order for s shells), with the shuffled
+    worst case beside it."""
+    from cipsi_amd import ijkl, synth
+    n = args.n_ao
+    B = synth.eri_factors(n, 3 * n, 20.0, 2026)
+    V = synth.eri_packed_torch(B, "cuda")
+    npk = V.numel()
+    ctx = ijkl.Context(0)
+    res = {}
+    for order in ("packed", "shuffled"):
+        xx, lab = labels_on_device(V, n, torch, shuffle=order == "shuffled")
+        torch.cuda.synchronize()
+        out = ctx.transd(xx, lab, n)
+        torch.cuda.synchronize()
+        exact = bool(torch.equal(out, V))
+        del out
+        ctx.enable_timing(True)
+        ctx.reset_timing()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = ctx.transd(xx, lab, n)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        tot, cnt = ctx.read_timing("unpack")
+        ctx.enable_timing(False)
+        res[order] = dict(ms=dt * 1e3, gbs=20.0 * npk / dt / 1e9, labels_per_s=npk / dt,
+                          event_ms=tot / cnt if cnt else None, exact_vs_packed_block=exact)
+        del xx, lab, out
+        torch.cuda.empty_cache()
+    p = res["packed"]
+    print(json.dumps({
+        "metric": "label ingest GB/s (transd on the GPU, N_AO=%d, pshf-order label stream)" % n,
+        "value": p["gbs"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": p["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+i32", "data": "synthetic labels of the packed AO block",
+        "config": {"workload": "transd C1 unpack, %d labels" % npk, "n_ao": n},
+        "labels_per_s": p["labels_per_s"], "exact_vs_packed_block": p["exact_vs_packed_block"],
+        "shuffled_stream": res["shuffled"],
+        "roofline": {"bound": "hbm", "kernel": "label unpack (find-first-zero, claim, store)",
+                     "achieved": round(p["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(p["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "event_ms": p["event_ms"]},
+    }))
+
+
 def run_scan(args, torch):
     """One step = the whole 64-geometry scan in one graph launch."""
     from cipsi_amd import ijkl, synth
@@ -227,7 +313,7 @@ def main():
     ap.add_argument("--n-mo", type=int, default=240)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--cpu-threads", type=int, default=1)
-    ap.add_argument("--workload", default="transform", choices=["transform", "scan"])
+    ap.add_argument("--workload", default="transform", choices=["transform", "scan", "labels"])
     ap.add_argument("--scan-variants", default="",
                     help='scan: fixed tile variants per row class "v1,v2;v1,v2;..." (no autotune)')
     ap.add_argument("--no-loop", action="store_true",
@@ -244,6 +330,9 @@ def main():
 
     if args.workload == "scan":
         run_scan(args, torch)
+        return
+    if args.workload == "labels":
+        run_labels(args, torch)
         return
 
     n, m = args.n_ao, args.n_mo
@@ -330,6 +419,14 @@ def main():
             "gpu_vs_cpu_sampled_maxabs": {"half1": d1, "half2": d2},
         }
         result["speedup_vs_cpu"] = value / (n_out / est)
+        ncores = min(16, os.cpu_count() or 1)
+        if ncores > 1:
+            est_all, spent_all, _, _, _, _ = cpu_baseline(ao_host, c_host, h_gpu, out_host, n, m,
+                                                          nthreads=ncores)
+            result["cpu_baseline_all_cores"] = {
+                "value": n_out / est_all, "unit": "MO-integrals/s", "cores": ncores,
+                "kind": "port", "sample": "same sample, OpenMP over AO pairs / MO pairs",
+                "est_seconds_per_transform": est_all}
     print(json.dumps(result))
 
 
