@@ -510,7 +510,7 @@ int cipsi_ijkl_tmono(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* h_pa
     return CIPSI_OK;
 }
 
-static void drain_timing(cipsi_ijkl_ctx* ctx) {
+extern "C++" void ctx_drain_timing(cipsi_ijkl_ctx* ctx) {
     for (auto& p : ctx->pending) {
         float ms = 0.f;
         if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
@@ -534,7 +534,7 @@ int cipsi_ijkl_enable_timing(cipsi_ijkl_ctx* ctx, int enable) {
 int cipsi_ijkl_reset_timing(cipsi_ijkl_ctx* ctx) {
     int rc = ctx_enter(ctx);
     if (rc) return rc;
-    drain_timing(ctx);
+    ctx_drain_timing(ctx);
     ctx->totals.clear();
     return CIPSI_OK;
 }
@@ -544,7 +544,7 @@ int cipsi_ijkl_read_timing(cipsi_ijkl_ctx* ctx, const char* stage, double* total
     int rc = ctx_enter(ctx);
     if (rc) return rc;
     if (!stage || !total_ms || !launches) return CIPSI_ERR_ARG;
-    drain_timing(ctx);
+    ctx_drain_timing(ctx);
     auto it = ctx->totals.find(stage);
     *total_ms = it == ctx->totals.end() ? 0.0 : it->second.first;
     *launches = it == ctx->totals.end() ? 0 : it->second.second;

@@ -47,6 +47,11 @@ struct cipsi_ijkl_ctx {
     std::map<std::string, std::pair<double, int64_t>> totals;
 };
 
+// Event pairs held before they are folded into the per-stage totals (bounds the memory
+// of a caller that enables timing and never reads it).
+constexpr size_t kMaxPendingTimers = 4096;
+void ctx_drain_timing(cipsi_ijkl_ctx* ctx);
+
 // Records HIP events around one stage launch on the context stream when timing is on.
 struct StageTimer {
     cipsi_ijkl_ctx* ctx;
@@ -62,6 +67,7 @@ struct StageTimer {
         if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
         (void)hipEventRecord(b, ctx->stream);
         ctx->pending.push_back({stage, a, b});
+        if (ctx->pending.size() >= kMaxPendingTimers) ctx_drain_timing(ctx);
     }
 };
 
