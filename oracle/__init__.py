@@ -50,6 +50,8 @@ def lib():
         L.or_energy_grad.restype = None
         L.or_energy_grad.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
                                      _dp, _dp, _dp, _dp]
+        L.or_fock.restype = ctypes.c_double
+        L.or_fock.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -127,6 +129,14 @@ def energy(mo_packed, hmo_packed, gamma, Gamma_packed):
     m = gamma.shape[0]
     return lib().or_energy(0, m, _d(_f64(mo_packed)), _d(_f64(hmo_packed)), _d(_f64(gamma)),
                            _d(_f64(Gamma_packed)))
+
+
+def fock(mo_packed, hmo_packed, n_mo, noc):
+    """fock.f:80-127 on the packed MO stream: (energy without enuc, F, J, K), packed."""
+    pm = n_pairs(n_mo)
+    f, j, k = np.empty(pm), np.empty(pm), np.empty(pm)
+    e = lib().or_fock(n_mo, noc, _d(_f64(mo_packed)), _d(_f64(hmo_packed)), _d(f), _d(j), _d(k))
+    return e, f, j, k
 
 
 def energy_grad(ao_packed, h_packed, c, gamma, Gamma_packed):

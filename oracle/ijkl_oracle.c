@@ -14,6 +14,7 @@
  *   or_tpqkld_c1   CIPSI_sources/ijkl/tpqkld.f:25-104  (first half, Q1 + Q2)
  *   or_tijkld_c1   CIPSI_sources/ijkl/tijkld.f:28-88   (second half, Q3 + Q4)
  *   or_tmono       CIPSI_sources/ijkl/tmono.f:39-54    (one-electron transform)
+ *   or_fock        CIPSI_sources/fock/fock.f:80-127    (closed-shell energy, Fock, J, K)
  *
  * Every accumulation starts from 0.0 and adds in the same index order as the
  * Fortran loop; build with -ffp-contract=off so that no multiply-add is fused.
@@ -391,4 +392,64 @@ void or_energy_grad(int n_ao, int n_mo, const double *bijkl, const double *hpk,
     free(h);
     free(out);
     free(hmo);
+}
+
+/*
+ * Integral consumer of the packed MO stream: the closed-shell reference energy and the
+ * MO Fock operator of CIPSI_sources/fock/fock.f:80-127, with the integrals fetched as
+ * ai() fetches them for C1 (fock/ai.f:12-35: pair indices, larger pair first).
+ * noc doubly occupied orbitals; hmo packed M(M+1)/2 (h_MO, i >= j); outputs packed:
+ *   fock[pair0(i,j)]  F_ij = h_ij + sum_{k<noc} 2 (ij|kk) - (ik|jk), then |F| < 1e-7 -> 0
+ *   jint[pair0(i,j)]  (ii|jj),  kint[pair0(i,j)]  (ij|ij)        (fock.f:108-112)
+ * The energy is val of fock.f:80-100 (without enuc).  Loop and summation order follow
+ * fock.f so the results match a build of the reference without FMA.
+ */
+static inline double or_ai(const double *mo, int i, int j, int k, int l)
+{
+    const int64_t ij = (i >= j) ? pair0(i, j) : pair0(j, i);
+    const int64_t kl = (k >= l) ? pair0(k, l) : pair0(l, k);
+    return (ij >= kl) ? mo[pair0(ij, kl)] : mo[pair0(kl, ij)];
+}
+
+double or_fock(int n_mo, int noc, const double *mo, const double *hmo, double *fock,
+               double *jint, double *kint)
+{
+    const int m = n_mo;
+    double *f = (double *)malloc(sizeof(double) * (size_t)m * m);   /* f(i,j), i >= j used */
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j <= i; ++j) f[(size_t)i * m + j] = hmo[pair0(i, j)];
+    double val = 0.0;
+    for (int i = 0; i < noc; ++i) {                                /* fock.f:81-99 */
+        val = val + f[(size_t)i * m + i] * 2.0;
+        double tt = or_ai(mo, i, i, i, i);
+        f[(size_t)i * m + i] = f[(size_t)i * m + i] + tt;
+        val = val + tt;
+        for (int k = 0; k < i; ++k) {
+            tt = or_ai(mo, i, i, k, k);
+            const double tk = or_ai(mo, i, k, i, k);
+            tt = tt + tt - tk;
+            f[(size_t)i * m + i] = f[(size_t)i * m + i] + tt;
+            f[(size_t)k * m + k] = f[(size_t)k * m + k] + tt;
+            val = val + tt + tt;
+            for (int j = 0; j < noc; ++j)
+                f[(size_t)i * m + k] = f[(size_t)i * m + k]
+                                       + (or_ai(mo, i, k, j, j) * 2.0 - or_ai(mo, i, j, k, j));
+        }
+    }
+    for (int i = noc; i < m; ++i)                                  /* fock.f:102-109 */
+        for (int j = 0; j <= i; ++j) {
+            double tt = 0.0;
+            for (int k = 0; k < noc; ++k)
+                tt = tt + (2.0 * or_ai(mo, i, j, k, k) - or_ai(mo, i, k, j, k));
+            f[(size_t)i * m + j] = f[(size_t)i * m + j] + tt;
+        }
+    for (int i = 0; i < m; ++i)                                    /* fock.f:110-114, 124-128 */
+        for (int j = 0; j <= i; ++j) {
+            jint[pair0(i, j)] = or_ai(mo, i, i, j, j);
+            kint[pair0(i, j)] = or_ai(mo, i, j, i, j);
+            const double v = f[(size_t)i * m + j];
+            fock[pair0(i, j)] = (fabs(v) < 1.0e-7) ? 0.0 : v;
+        }
+    free(f);
+    return val;
 }
