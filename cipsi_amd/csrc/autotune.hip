@@ -100,7 +100,7 @@ int cipsi_ijkl_autotune(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, int precision, 
     if (n_ao > 256) return CIPSI_ERR_UNSUPPORTED;
     if (precision != CIPSI_PREC_FP64) return CIPSI_ERR_UNSUPPORTED;
     const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
-    const int np = (n_ao + 15) & ~15, mp = (n_mo + 15) & ~15;
+    const int mp = (n_mo + 15) & ~15;
     if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->h, sizeof(double) * pa * pm))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->io_out, sizeof(double) * npairs(pm)))) return rc;

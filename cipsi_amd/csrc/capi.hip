@@ -237,6 +237,7 @@ int cipsi_ijkl_release_workspace(cipsi_ijkl_ctx* ctx) {
     ctx_free(ctx->adj_g);
     ctx_free(ctx->adj_gh);
     ctx_free(ctx->adj_small);
+    ctx_free(ctx->cons);
     ctx->ts_m = -1;
     return CIPSI_OK;
 }

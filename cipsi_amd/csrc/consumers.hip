@@ -1,0 +1,234 @@
+// Integral consumers of the packed MO stream, on the device.
+//
+//   gather   ai(i,j,k,l) for a batch of integral labels: the random access that the
+//            downstream programs make into the MO integrals (fock/ai.f:12-35, C1 route:
+//            pair indices, larger pair first; cip/ai.f:1-42 for the same lookup).
+//   fock     the closed-shell reference energy, the MO Fock operator and the J / K
+//            diagonals of CIPSI_sources/fock/fock.f:80-127.
+//
+// Conventions (0-based here, the reference is 1-based):
+//   mo[pair0(max(a,b), min(a,b))], a = pair(i,j), b = pair(k,l)   the packed MO stream
+//   hmo[pair0(i,j)], i >= j                                        h_MO (tmono.f:47-54)
+//   label = i<<24 | j<<16 | k<<8 | l, 1-based 8-bit fields       (ijkl/iword_func.f:2-52)
+//
+// The Fock elements and the energy are summed in exactly the order of fock.f, one element
+// per thread, with multiply-add contraction off: on the same MO integrals the results are
+// the bits of a build of fock.f without FMA.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/cipsi_ijkl.h"
+#include "common.h"
+#include "ctx.h"
+
+using namespace cipsi;
+
+namespace {
+
+// fock.f:3420: elements below this magnitude are set to zero.
+constexpr double kFockThreshold = 1.0e-7;
+constexpr int kEnergyThreads = 256;
+constexpr int kMaxOcc = 256;
+
+__device__ inline double ai(const double* __restrict__ mo, int i, int j, int k, int l) {
+    return mo[pairsym(pairsym(i, j), pairsym(k, l))];
+}
+
+// One thread per label; out-of-range fields give NaN and are counted.
+__global__ void k_gather(const double* __restrict__ mo, int m, const int32_t* __restrict__ lab,
+                         int64_t n, double* __restrict__ out,
+                         unsigned long long* __restrict__ n_bad) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        const uint32_t u = (uint32_t)lab[q];
+        const int i = (int)(u >> 24), j = (int)((u >> 16) & 255u), k = (int)((u >> 8) & 255u),
+                  l = (int)(u & 255u);
+        const bool ok = i >= 1 && j >= 1 && k >= 1 && l >= 1 && i <= m && j <= m && k <= m &&
+                        l <= m;
+        if (ok) {
+            out[q] = ai(mo, i - 1, j - 1, k - 1, l - 1);
+        } else {
+            out[q] = __builtin_nan("");
+            atomicAdd(n_bad, 1ull);
+        }
+    }
+}
+
+// fock.f:81-127 per element (i >= j), in the reference's order of additions:
+//   i < noc, j < i   f = h; for k < noc: f = f + (2 (ij|kk) - (ik|jk))           (:87-96)
+//   i < noc, j = i   f = h + (ii|ii); then tt(i,k) = 2(ii|kk) - (ik|ik) for k < i, then
+//                    for k = i+1 .. noc-1 (added when the outer loop reaches k, :85-93)
+//   i >= noc         tt = 0; for k < noc: tt = tt + (2 (ij|kk) - (ik|jk)); f = h + tt  (:102-109)
+// then |f| < 1e-7 -> 0 (:3420), and J = (ii|jj), K = (ij|ij) (:110-114).
+__global__ void k_fock_elements(const double* __restrict__ mo, const double* __restrict__ hmo,
+                                int m, int noc, double* __restrict__ fock,
+                                double* __restrict__ jint, double* __restrict__ kint) {
+#pragma clang fp contract(off)
+    const int64_t pm = pair0(m, 0);
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= pm) return;
+    const int i = (int)pair_row(x);
+    const int j = (int)(x - pair0(i, 0));
+    const double h = hmo[x];
+    double f;
+    if (i >= noc) {
+        double tt = 0.0;
+        for (int k = 0; k < noc; ++k) tt = tt + (2.0 * ai(mo, i, j, k, k) - ai(mo, i, k, j, k));
+        f = h + tt;
+    } else if (j < i) {
+        f = h;
+        for (int k = 0; k < noc; ++k) f = f + (ai(mo, i, j, k, k) * 2.0 - ai(mo, i, k, j, k));
+    } else {
+        f = h + ai(mo, i, i, i, i);
+        for (int k = 0; k < noc; ++k) {
+            if (k == i) continue;
+            double tt = ai(mo, i, i, k, k);
+            const double tk = ai(mo, i, k, i, k);
+            tt = tt + tt - tk;
+            f = f + tt;
+        }
+    }
+    fock[x] = fabs(f) < kFockThreshold ? 0.0 : f;
+    jint[x] = ai(mo, i, i, j, j);
+    kint[x] = ai(mo, i, j, i, j);
+}
+
+// fock.f:80-99: val, the closed-shell electronic energy (without the nuclear term), one
+// work-group.  The terms of row i are formed in parallel and then added by one lane in the
+// reference's order: 2 h_ii, (ii|ii), then tt(i,k) twice for each k < i.
+__global__ void __launch_bounds__(kEnergyThreads)
+k_fock_energy(const double* __restrict__ mo, const double* __restrict__ hmo, int noc,
+              double* __restrict__ energy) {
+#pragma clang fp contract(off)
+    __shared__ double tt[kMaxOcc];
+    double val = 0.0;
+    for (int i = 0; i < noc; ++i) {
+        for (int k = threadIdx.x; k < i; k += blockDim.x) {
+            double t = ai(mo, i, i, k, k);
+            const double tk = ai(mo, i, k, i, k);
+            tt[k] = t + t - tk;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            val = val + hmo[pair0(i, i)] * 2.0;
+            val = val + ai(mo, i, i, i, i);
+            for (int k = 0; k < i; ++k) val = val + tt[k] + tt[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *energy = val;
+}
+
+int cons_scalars(cipsi_ijkl_ctx* ctx, unsigned long long** n_bad, double** energy) {
+    int rc = ctx_alloc(ctx, ctx->cons, 64);
+    if (rc) return rc;
+    *n_bad = (unsigned long long*)ctx->cons.ptr;
+    *energy = (double*)((char*)ctx->cons.ptr + 16);
+    return CIPSI_OK;
+}
+
+int gather_impl(cipsi_ijkl_ctx* ctx, int m, const double* d_mo, const int32_t* d_labels,
+                int64_t n, double* d_out) {
+    unsigned long long* n_bad;
+    double* e;
+    int rc = cons_scalars(ctx, &n_bad, &e);
+    if (rc) return rc;
+    if (hipMemsetAsync(n_bad, 0, sizeof(unsigned long long), ctx->stream) != hipSuccess)
+        return CIPSI_ERR_HIP;
+    if (n > 0) {
+        StageTimer tm(ctx, "gather");
+        const int64_t blocks = (n + 255) / 256;
+        const unsigned grid = (unsigned)(blocks < 65536 ? blocks : 65536);
+        hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, ctx->stream, d_mo, m, d_labels, n,
+                           d_out, n_bad);
+        if (hipGetLastError() != hipSuccess) return CIPSI_ERR_HIP;
+    }
+    unsigned long long bad = 0;
+    if (hipMemcpyAsync(&bad, n_bad, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return CIPSI_ERR_HIP;
+    if (bad) {
+        ctx->last_error = "gather: " + std::to_string(bad) + " labels with a field outside 1..n_mo";
+        return CIPSI_ERR_LABEL;
+    }
+    return CIPSI_OK;
+}
+
+int fock_impl(cipsi_ijkl_ctx* ctx, int m, int noc, const double* d_mo, const double* d_hmo,
+              double* d_energy, double* d_fock, double* d_j, double* d_k) {
+    StageTimer tm(ctx, "fock");
+    const int64_t pm = pair0(m, 0);
+    hipLaunchKernelGGL(k_fock_elements, dim3((unsigned)((pm + 63) / 64)), dim3(64), 0, ctx->stream,
+                       d_mo, d_hmo, m, noc, d_fock, d_j, d_k);
+    hipLaunchKernelGGL(k_fock_energy, dim3(1), dim3(kEnergyThreads), 0, ctx->stream, d_mo, d_hmo,
+                       noc, d_energy);
+    return hipGetLastError() == hipSuccess ? CIPSI_OK : CIPSI_ERR_HIP;
+}
+
+int check_fock_args(int m, int noc) {
+    if (m < 1 || noc < 0 || noc > m) return CIPSI_ERR_ARG;
+    if (m > kMaxOcc) return CIPSI_ERR_UNSUPPORTED;
+    return CIPSI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cipsi_ijkl_gather_dev(cipsi_ijkl_ctx* ctx, int n_mo, const double* d_mo_packed,
+                          const int32_t* d_labels, int64_t n, double* d_out) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if (n_mo < 1 || n < 0 || !d_mo_packed || (n > 0 && (!d_labels || !d_out)))
+        return CIPSI_ERR_ARG;
+    if (n_mo > 255) return CIPSI_ERR_LABEL;
+    return gather_impl(ctx, n_mo, d_mo_packed, d_labels, n, d_out);
+}
+
+int cipsi_ijkl_fock_dev(cipsi_ijkl_ctx* ctx, int n_mo, int noc, const double* d_mo_packed,
+                        const double* d_h_mo_packed, double* d_energy, double* d_fock,
+                        double* d_j, double* d_k) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_fock_args(n_mo, noc))) return rc;
+    if (!d_mo_packed || !d_h_mo_packed || !d_energy || !d_fock || !d_j || !d_k)
+        return CIPSI_ERR_ARG;
+    return fock_impl(ctx, n_mo, noc, d_mo_packed, d_h_mo_packed, d_energy, d_fock, d_j, d_k);
+}
+
+int cipsi_ijkl_fock(cipsi_ijkl_ctx* ctx, int n_mo, int noc, const double* mo_packed,
+                    const double* h_mo_packed, double* energy, double* fock, double* jint,
+                    double* kint) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_fock_args(n_mo, noc))) return rc;
+    if (!mo_packed || !h_mo_packed || !energy || !fock || !jint || !kint) return CIPSI_ERR_ARG;
+    const int64_t pm = pair0(n_mo, 0);
+    const size_t s_mo = sizeof(double) * (size_t)pair0(pm, 0), s_p = sizeof(double) * pm;
+    if ((rc = ctx_alloc(ctx, ctx->io_in, s_mo))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->io_in2, s_p))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->io_out, 3 * s_p))) return rc;
+    unsigned long long* n_bad;
+    double* d_e;
+    if ((rc = cons_scalars(ctx, &n_bad, &d_e))) return rc;
+    double* d_f = (double*)ctx->io_out.ptr;
+    double* d_j = d_f + pm;
+    double* d_k = d_j + pm;
+    hipStream_t st = ctx->stream;
+    if (hipMemcpyAsync(ctx->io_in.ptr, mo_packed, s_mo, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(ctx->io_in2.ptr, h_mo_packed, s_p, hipMemcpyHostToDevice, st) != hipSuccess)
+        return CIPSI_ERR_HIP;
+    if ((rc = fock_impl(ctx, n_mo, noc, (const double*)ctx->io_in.ptr,
+                        (const double*)ctx->io_in2.ptr, d_e, d_f, d_j, d_k)))
+        return rc;
+    if (hipMemcpyAsync(energy, d_e, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(fock, d_f, s_p, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(jint, d_j, s_p, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(kint, d_k, s_p, hipMemcpyDeviceToHost, st) != hipSuccess)
+        return CIPSI_ERR_HIP;
+    return hipStreamSynchronize(st) == hipSuccess ? CIPSI_OK : CIPSI_ERR_HIP;
+}
+
+}  // extern "C"

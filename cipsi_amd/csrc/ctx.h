@@ -33,6 +33,7 @@ struct cipsi_ijkl_ctx {
     DevBuf adj_g;    // energy/gradient: packed MO integrals
     DevBuf adj_gh;   // energy/gradient: scaled pair density Gh[b][a] = m(b) Gamma(a,b)
     DevBuf adj_small;  // energy/gradient: h_MO, hC, reduction partials
+    DevBuf cons;     // integral consumers: bad-label count, energy
     void* blas = nullptr;  // rocblas_handle, created on first gradient call
     int ts_m = -1, ts_lt = -1;
     int64_t ts_total = 0;

@@ -77,6 +77,17 @@ module cipsi_ijkl_mod
       real(c_double), intent(in) :: ao_packed(*), h_packed(*), c(*), gamma(*), Gamma_packed(*)
       real(c_double), intent(out) :: energy, grad(*)
     end function
+
+    ! Closed-shell energy (no nuclear term), Fock (|F| < 1e-7 -> 0), J and K of
+    ! fock.f:80-127 for noc doubly occupied MOs; all packed i >= j like h_mo_packed.
+    integer(c_int) function cipsi_ijkl_fock(ctx, n_mo, noc, mo_packed, h_mo_packed, energy, &
+        fock, jint, kint) bind(C, name="cipsi_ijkl_fock")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n_mo, noc
+      real(c_double), intent(in) :: mo_packed(*), h_mo_packed(*)
+      real(c_double), intent(out) :: energy, fock(*), jint(*), kint(*)
+    end function
   end interface
 
 contains

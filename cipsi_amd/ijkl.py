@@ -48,6 +48,7 @@ EXPORTS = [
     "cipsi_ijkl_batch_workspace_bytes", "cipsi_ijkl_batch_create", "cipsi_ijkl_batch_destroy",
     "cipsi_ijkl_batch_autotune", "cipsi_ijkl_batch_run", "cipsi_ijkl_batch_num_classes",
     "cipsi_ijkl_batch_class_info", "cipsi_ijkl_batch_set_class_plan",
+    "cipsi_ijkl_gather_dev", "cipsi_ijkl_fock", "cipsi_ijkl_fock_dev",
 ]
 
 
@@ -119,6 +120,9 @@ def load_library(path: str = LIB_PATH):
     L.cipsi_ijkl_batch_num_classes.argtypes = [_vp]
     L.cipsi_ijkl_batch_class_info.argtypes = [_vp, _c_int, _ip, _ip, _ip, _ip]
     L.cipsi_ijkl_batch_set_class_plan.argtypes = [_vp, _vp, _c_int, _c_int, _c_int]
+    L.cipsi_ijkl_gather_dev.argtypes = [_vp, _c_int, _vp, _vp, _c_i64, _vp]
+    L.cipsi_ijkl_fock.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.cipsi_ijkl_fock_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]
     _lib = L
     return L
 
@@ -356,6 +360,56 @@ class Context:
             self.handle, n, m, _ptr(arrs[0]), _ptr(arrs[1]), _ptr(cm), ldc, _ptr(arrs[2]),
             _ptr(arrs[3]), ctypes.byref(e), _ptr(g) if want_grad else None), "energy_grad")
         return e.value, g
+
+    def gather(self, mo_packed, n_mo, labels):
+        """ai(i,j,k,l) for each label (1-based MO indices in the 8-bit fields of the label
+        codec), gathered from the packed MO stream on the device (fock/ai.f:12-35).
+        mo_packed: float64 CUDA tensor; labels: int32 CUDA tensor.  Returns float64."""
+        import torch
+        if not (_is_torch(mo_packed) and mo_packed.is_cuda and mo_packed.dtype == torch.float64
+                and mo_packed.numel() == n_pairs(n_pairs(n_mo))):
+            raise ValueError("gather: mo_packed must be a float64 CUDA tensor of P_M(P_M+1)/2")
+        if not (_is_torch(labels) and labels.is_cuda and labels.dtype == torch.int32):
+            raise ValueError("gather: labels must be an int32 CUDA tensor")
+        mo_packed = mo_packed.contiguous()
+        labels = labels.contiguous().view(-1)
+        out = torch.empty(labels.numel(), dtype=torch.float64, device=labels.device)
+        self._use_torch_stream()
+        self._check(self.lib.cipsi_ijkl_gather_dev(self.handle, n_mo, _ptr(mo_packed),
+                                                   _ptr(labels), labels.numel(), _ptr(out)),
+                    "gather")
+        return out
+
+    def fock(self, mo_packed, h_mo_packed, n_mo, noc):
+        """Closed-shell energy (without the nuclear term), Fock, J and K (packed) of
+        fock.f:80-127 for noc doubly occupied MOs.  Torch CUDA tensors run on torch's
+        current stream and return tensors (energy as a 0-d tensor); numpy arrays use the
+        synchronous host entry point."""
+        pm = n_pairs(n_mo)
+        if _is_torch(mo_packed):
+            import torch
+            args = [x.contiguous() for x in (mo_packed, h_mo_packed)]
+            for x, size in zip(args, (n_pairs(pm), pm)):
+                if x.dtype != torch.float64 or not x.is_cuda or x.numel() != size:
+                    raise ValueError("fock: inputs must be float64 CUDA tensors of the "
+                                     "documented sizes")
+            dev = mo_packed.device
+            e = torch.empty(1, dtype=torch.float64, device=dev)
+            f, j, k = (torch.empty(pm, dtype=torch.float64, device=dev) for _ in range(3))
+            self._use_torch_stream()
+            self._check(self.lib.cipsi_ijkl_fock_dev(self.handle, n_mo, noc, _ptr(args[0]),
+                                                     _ptr(args[1]), _ptr(e), _ptr(f), _ptr(j),
+                                                     _ptr(k)), "fock")
+            return e[0], f, j, k
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (mo_packed, h_mo_packed)]
+        if arrs[0].size != n_pairs(pm) or arrs[1].size != pm:
+            raise ValueError("fock: input of the wrong size")
+        e = ctypes.c_double()
+        f, j, k = (np.zeros(pm) for _ in range(3))
+        self.lib.cipsi_ijkl_reset_stream(self.handle)
+        self._check(self.lib.cipsi_ijkl_fock(self.handle, n_mo, noc, _ptr(arrs[0]), _ptr(arrs[1]),
+                                             ctypes.byref(e), _ptr(f), _ptr(j), _ptr(k)), "fock")
+        return e.value, f, j, k
 
 
 class BatchPlan:

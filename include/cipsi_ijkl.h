@@ -199,6 +199,28 @@ int cipsi_ijkl_batch_class_info(const cipsi_ijkl_batch *plan, int k, int *row_bl
 int cipsi_ijkl_batch_set_class_plan(cipsi_ijkl_ctx *ctx, cipsi_ijkl_batch *plan, int k,
                                     int var_half1, int var_half2);
 
+/* ---- Integral consumers of the packed MO stream ----------------------------------
+ * gather: out[q] = ai(i,j,k,l) for the 1-based MO indices of labels[q] (the label codec
+ *   of ijkl/iword_func.f:2-52, i<<24 | j<<16 | k<<8 | l); the lookup of fock/ai.f:12-35
+ *   for C1, pair indices with the larger pair first, for any index order.  Device
+ *   pointers; n_mo <= 255.  A label with a field outside 1..n_mo gives NaN and the call
+ *   returns CIPSI_ERR_LABEL.  Synchronous at return (it reads back that count).
+ * fock: the closed-shell electronic energy (without the nuclear term), the MO Fock
+ *   operator with |F| < 1e-7 set to zero, and the J = (ii|jj) and K = (ij|ij) matrices of
+ *   CIPSI_sources/fock/fock.f:80-127, for noc doubly occupied MOs (the first noc).
+ *   h_mo_packed is tmono's output; fock, j, k are packed like it (i >= j).  Sums run in
+ *   fock.f's order without fused multiply-adds, so on equal MO integrals the results
+ *   equal those of fock.f built without FMA.  The _dev form is asynchronous on the
+ *   context stream; the host form copies in and out and is synchronous. */
+int cipsi_ijkl_gather_dev(cipsi_ijkl_ctx *ctx, int n_mo, const double *d_mo_packed,
+                          const int32_t *d_labels, int64_t n, double *d_out);
+int cipsi_ijkl_fock(cipsi_ijkl_ctx *ctx, int n_mo, int noc, const double *mo_packed,
+                    const double *h_mo_packed, double *energy, double *fock, double *jint,
+                    double *kint);
+int cipsi_ijkl_fock_dev(cipsi_ijkl_ctx *ctx, int n_mo, int noc, const double *d_mo_packed,
+                        const double *d_h_mo_packed, double *d_energy, double *d_fock,
+                        double *d_j, double *d_k);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,149 @@
+"""Integral consumers on the device (cipsi_ijkl_gather_dev, cipsi_ijkl_fock[_dev]).
+
+* fock: on the same packed MO stream, the library's energy, Fock, J and K equal the
+  oracle's or_fock bit for bit (both sum in fock.f's order without FMA); or_fock is pinned
+  to the reference fock program and to a dense formula by tests/test_oracle_fock.py.
+  Edge cases noc = 0, 1 and noc = n_mo; the host and device entry points agree.
+* end to end at N_AO = N_MO = 240 (the 418,197,660-integral transform): the GPU transform
+  feeds the GPU Fock build; energy, Fock, J and K against or_fock on that same stream, and
+  the closed-shell energy and the virtual-virtual Fock block are invariant under a
+  rotation among the occupied MOs (fock.f:80-101 depends on the occupied space only).
+* gather: ai(i,j,k,l) for labels in any index order, against a numpy lookup of the packed
+  stream with the larger pair first (fock/ai.f:12-35); labels outside 1..n_mo are refused.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cipsi_amd import formats, synth
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from cipsi_amd import ijkl
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = ijkl.Context(0)
+    yield c
+    c.close()
+
+
+def _pairsym(a, b):
+    hi, lo = np.maximum(a, b), np.minimum(a, b)
+    return hi * (hi + 1) // 2 + lo
+
+
+def _lookup(mo, i, j, k, l):
+    """ai() on the packed stream with 0-based indices (numpy)."""
+    return mo[_pairsym(_pairsym(i, j), _pairsym(k, l))]
+
+
+def _small_case(n, noc, seed):
+    case = synth.make_case(n, n, 2 * n, 4.0, seed)
+    mo = oracle.ijkl(case["ao_packed"], case["c_full"])
+    hmo = oracle.tmono(synth.symmetric_packed(n, seed + 1), case["c_full"])
+    return mo, hmo
+
+
+@pytest.mark.parametrize("n,noc", [(16, 5), (40, 10), (23, 0), (23, 1), (19, 19)])
+def test_fock_bitwise_equal_to_oracle(ctx, n, noc):
+    print(f"[fock] n={n} noc={noc}", flush=True)
+    mo, hmo = _small_case(n, noc, 900 + n + noc)
+    e, f, j, k = ctx.fock(torch.as_tensor(mo, device="cuda"), torch.as_tensor(hmo, device="cuda"),
+                          n, noc)
+    torch.cuda.synchronize()
+    re, rf, rj, rk = oracle.fock(mo, hmo, n, noc)
+    assert float(e) == re
+    np.testing.assert_array_equal(f.cpu().numpy(), rf)
+    np.testing.assert_array_equal(j.cpu().numpy(), rj)
+    np.testing.assert_array_equal(k.cpu().numpy(), rk)
+    # the host entry point gives the same bits
+    he, hf, hj, hk = ctx.fock(mo, hmo, n, noc)
+    assert he == re
+    np.testing.assert_array_equal(hf, rf)
+    np.testing.assert_array_equal(hj, rj)
+    np.testing.assert_array_equal(hk, rk)
+
+
+def test_fock_rejects_bad_input(ctx):
+    from cipsi_amd import ijkl
+    mo, hmo = _small_case(8, 2, 950)
+    with pytest.raises(ijkl.IjklError):
+        ctx.fock(mo, hmo, 8, 9)                       # noc > n_mo
+    with pytest.raises(ValueError):
+        ctx.fock(mo[:-1], hmo, 8, 2)
+
+
+def test_gather_matches_packed_lookup(ctx):
+    from cipsi_amd import ijkl
+    n = 37
+    mo, _ = _small_case(n, 0, 960)
+    rng = np.random.default_rng(961)
+    q = rng.integers(1, n + 1, size=(4, 100_003))
+    q[:, :4] = [[1, 1, n, n], [1, n, 1, n], [n, 1, 1, n], [1, 1, 1, 1]]   # extremes, all orders
+    labels = formats.label_encode(*q)
+    d_mo = torch.as_tensor(mo, device="cuda")
+    out = ctx.gather(d_mo, n, torch.as_tensor(labels, device="cuda"))
+    torch.cuda.synchronize()
+    want = _lookup(mo, *(q - 1))
+    np.testing.assert_array_equal(out.cpu().numpy(), want)
+    # every permutation symmetry of (ij|kl) gives the same value
+    i, j, k, l = q
+    for perm in ((j, i, k, l), (i, j, l, k), (k, l, i, j), (l, k, j, i)):
+        o2 = ctx.gather(d_mo, n, torch.as_tensor(formats.label_encode(*perm), device="cuda"))
+        assert torch.equal(o2, out)
+    # empty batch
+    assert ctx.gather(d_mo, n, torch.empty(0, dtype=torch.int32, device="cuda")).numel() == 0
+    # a field outside 1..n_mo is refused
+    bad = labels.copy()
+    bad[5] = formats.label_encode(n + 1, 1, 1, 1)
+    with pytest.raises(ijkl.IjklError):
+        ctx.gather(d_mo, n, torch.as_tensor(bad, device="cuda"))
+    bad[5] = formats.label_encode(3, 0, 1, 1)
+    with pytest.raises(ijkl.IjklError):
+        ctx.gather(d_mo, n, torch.as_tensor(bad, device="cuda"))
+
+
+def test_fock_after_full_size_transform(ctx):
+    """N_AO = N_MO = 240, rank 720, decay 20 (the 418,197,660-integral transform), 60
+    doubly occupied MOs."""
+    n, noc = 240, 60
+    print(f"[fock-full] n={n} noc={noc}", flush=True)
+    B = synth.eri_factors(n, 3 * n, 20.0, 2024)
+    ao = synth.eri_packed_torch(B, "cuda")
+    c = synth.orthonormal(n, 2025)
+    h = synth.symmetric_packed(n, 2026)
+    mo = ctx.transform(ao, torch.as_tensor(c, device="cuda"))
+    hmo = ctx.tmono(h, c)
+    e, f, j, k = ctx.fock(mo, torch.as_tensor(hmo, device="cuda"), n, noc)
+    torch.cuda.synchronize()
+    mo_h = mo.cpu().numpy()
+    re, rf, rj, rk = oracle.fock(mo_h, hmo, n, noc)
+    assert float(e) == re
+    np.testing.assert_array_equal(f.cpu().numpy(), rf)
+    np.testing.assert_array_equal(j.cpu().numpy(), rj)
+    np.testing.assert_array_equal(k.cpu().numpy(), rk)
+    # gather of 1e7 random labels from the full stream
+    rng = np.random.default_rng(2027)
+    q = rng.integers(1, n + 1, size=(4, 10_000_000))
+    out = ctx.gather(mo, n, torch.as_tensor(formats.label_encode(*q), device="cuda"))
+    np.testing.assert_array_equal(out.cpu().numpy(), _lookup(mo_h, *(q - 1)))
+    del mo_h
+    # rotate the occupied MOs among themselves
+    r = synth.orthonormal(noc, 2028)
+    c2 = c.copy()
+    c2[:, :noc] = c[:, :noc] @ r
+    del mo
+    mo2 = ctx.transform(ao, torch.as_tensor(c2, device="cuda"))
+    hmo2 = ctx.tmono(h, c2)
+    e2, f2, _, _ = ctx.fock(mo2, torch.as_tensor(hmo2, device="cuda"), n, noc)
+    torch.cuda.synchronize()
+    e, e2 = float(e), float(e2)
+    print(f"[fock-full] energy {e!r} rotated {e2!r}", flush=True)
+    assert abs(e2 - e) <= 1e-11 * abs(e)
+    fv = synth.unpack_sym(f.cpu().numpy(), n)[noc:, noc:]
+    fv2 = synth.unpack_sym(f2.cpu().numpy(), n)[noc:, noc:]
+    assert np.max(np.abs(fv2 - fv)) <= 1e-11 * np.max(np.abs(fv))
