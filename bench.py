@@ -21,8 +21,11 @@ autotuned tile variants per row class; reports geometries/s, the FP64 MFMA flop 
 of the whole launch against the matrix peak, and the same scan run as a loop of
 single transforms for comparison.
 
+Label ingest (--workload labels) and integral consumers (--workload consumers: the ai()
+gather and the Fock build on the transformed stream) print lines of their own.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|split_fp16|both]
-                       [--workload transform|scan]
+                       [--workload transform|scan|labels|consumers]
 """
 from __future__ import annotations
 
@@ -44,6 +47,9 @@ FP64_PEAK_TFLOPS = 78.6
 # WRITE_SIZE, factors from a 1 GiB copy in the same process); keyed by
 # (n_ao, n_mo, precision).  Summaries: profiles/fp64_n240_kernels.md.
 MEASURED_TRAFFIC = {(240, 240, "fp64"): (2.0 * 3782600.0 + 6786500.0) * 1024}
+# k_gather over 2^27 uniformly random labels at N_MO = 240, per launch: 2.0 x FETCH_SIZE
+# 8.6412e6 KiB + 1.0 x WRITE_SIZE 1.0486e6 KiB (profiles/consumers_n240.md).
+MEASURED_TRAFFIC_GATHER = {(240, 1 << 27): (2.0 * 8.6412e6 + 1.0486e6) * 1024}
 FP16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 
@@ -224,6 +230,88 @@ def run_labels(args, torch):
     }))
 
 
+def run_consumers(args, torch):
+    """Integral consumers on the device after one N_MO = 240 transform: the batched ai()
+    gather over 2^27 uniformly random labels (HBM random access: 4 B label + 8 B
+    integral read + 8 B written per lookup, algorithmic) and the closed-shell Fock build
+    of fock.f:80-127 with N_MO / 4 doubly occupied MOs, beside the oracle's serial Fock
+    build on one host core."""
+    import oracle
+    from cipsi_amd import formats, ijkl, synth
+    n = m = args.n_mo
+    noc = m // 4
+    B = synth.eri_factors(n, 3 * n, 20.0, 2026)
+    ao = synth.eri_packed_torch(B, "cuda")
+    c = synth.orthonormal(n, 2027)
+    h = synth.symmetric_packed(n, 2028)
+    ctx = ijkl.Context(0)
+    mo = ctx.transform(ao, torch.as_tensor(c, device="cuda"))
+    del ao
+    hmo = ctx.tmono(h, c)
+    d_hmo = torch.as_tensor(hmo, device="cuda")
+    n_q = 1 << 27
+    gen = torch.Generator(device="cuda").manual_seed(2029)
+    q = torch.randint(1, m + 1, (4, n_q), device="cuda", dtype=torch.int32, generator=gen)
+    labels = (q[0] << 24) | (q[1] << 16) | (q[2] << 8) | q[3]
+    del q
+    out = ctx.gather(mo, m, labels)
+    for _ in range(args.warmup):
+        ctx.fock(mo, d_hmo, m, noc)
+    torch.cuda.synchronize()
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = ctx.gather(mo, m, labels)
+    torch.cuda.synchronize()
+    dt_g = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e, f, j, k = ctx.fock(mo, d_hmo, m, noc)
+    torch.cuda.synchronize()
+    dt_f = (time.perf_counter() - t0) / args.steps
+    g_tot, g_cnt = ctx.read_timing("gather")
+    f_tot, f_cnt = ctx.read_timing("fock")
+    ctx.enable_timing(False)
+    g_ms, f_ms = g_tot / g_cnt, f_tot / f_cnt
+    gbs = 20.0 * n_q / (g_ms * 1e-3) / 1e9
+    traffic = MEASURED_TRAFFIC_GATHER.get((m, n_q))
+    res = {
+        "metric": "integral lookups/s (ai() gather from the N_MO=%d packed MO stream)" % m,
+        "value": n_q / (g_ms * 1e-3), "unit": "lookups/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": g_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+i32",
+        "data": "synthetic: GPU-transformed MO stream, uniform random labels",
+        "config": {"workload": "ai() gather, %d labels, N_MO=%d" % (n_q, m), "n_mo": m},
+        "host_ms_per_gather": dt_g * 1e3,
+        "roofline": {"bound": "hbm", "kernel": "k_gather", "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "traffic_gbs": (round(traffic / (g_ms * 1e-3) / 1e9, 1) if traffic
+                                     else None)},
+        "fock": {"n_mo": m, "noc": noc, "event_ms": f_ms, "host_ms": dt_f * 1e3,
+                 "energy": float(e)},
+    }
+    if not args.no_cpu:
+        mo_h = mo.cpu().numpy()
+        t0 = time.perf_counter()
+        re, rf, rj, rk = oracle.fock(mo_h, hmo, m, noc)
+        dt_c = time.perf_counter() - t0
+        res["fock"]["cpu_baseline"] = {"ms": dt_c * 1e3, "cores": 1, "kind": "port",
+                                       "sample": "the whole Fock build (oracle or_fock)"}
+        res["fock"]["bitwise_equal_to_oracle"] = bool(
+            float(e) == re and np.array_equal(f.cpu().numpy(), rf)
+            and np.array_equal(j.cpu().numpy(), rj) and np.array_equal(k.cpu().numpy(), rk))
+        sel = torch.arange(0, n_q, 4099, device="cuda")
+        lab_s = labels[sel].cpu().numpy().astype(np.int32)
+        i, jj, kk, ll = (x - 1 for x in formats.label_decode(lab_s))
+        pij = np.maximum(i, jj) * (np.maximum(i, jj) + 1) // 2 + np.minimum(i, jj)
+        pkl = np.maximum(kk, ll) * (np.maximum(kk, ll) + 1) // 2 + np.minimum(kk, ll)
+        pos = np.maximum(pij, pkl) * (np.maximum(pij, pkl) + 1) // 2 + np.minimum(pij, pkl)
+        res["gather_sample_exact"] = bool(np.array_equal(out[sel].cpu().numpy(), mo_h[pos]))
+    print(json.dumps(res))
+
+
 def run_scan(args, torch):
     """One step = the whole 64-geometry scan in one graph launch."""
     from cipsi_amd import ijkl, synth
@@ -313,7 +401,7 @@ def main():
     ap.add_argument("--n-mo", type=int, default=240)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--cpu-threads", type=int, default=1)
-    ap.add_argument("--workload", default="transform", choices=["transform", "scan", "labels"])
+    ap.add_argument("--workload", default="transform", choices=["transform", "scan", "labels", "consumers"])
     ap.add_argument("--scan-variants", default="",
                     help='scan: fixed tile variants per row class "v1,v2;v1,v2;..." (no autotune)')
     ap.add_argument("--no-loop", action="store_true",
@@ -333,6 +421,9 @@ def main():
         return
     if args.workload == "labels":
         run_labels(args, torch)
+        return
+    if args.workload == "consumers":
+        run_consumers(args, torch)
         return
 
     n, m = args.n_ao, args.n_mo

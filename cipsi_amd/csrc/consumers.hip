@@ -30,7 +30,8 @@ namespace {
 // fock.f:3420: elements below this magnitude are set to zero.
 constexpr double kFockThreshold = 1.0e-7;
 constexpr int kEnergyThreads = 256;
-constexpr int kMaxOcc = 256;
+constexpr int kMaxMo = 256;       // n_mo cap of the library (8-bit labels cap gather at 255)
+constexpr int kEnergyStage = 8192;   // doubles of tt staged in LDS per pass (64 KiB)
 
 __device__ inline double ai(const double* __restrict__ mo, int i, int j, int k, int l) {
     return mo[pairsym(pairsym(i, j), pairsym(k, l))];
@@ -75,10 +76,12 @@ __global__ void k_fock_elements(const double* __restrict__ mo, const double* __r
     double f;
     if (i >= noc) {
         double tt = 0.0;
+#pragma unroll 4
         for (int k = 0; k < noc; ++k) tt = tt + (2.0 * ai(mo, i, j, k, k) - ai(mo, i, k, j, k));
         f = h + tt;
     } else if (j < i) {
         f = h;
+#pragma unroll 4
         for (int k = 0; k < noc; ++k) f = f + (ai(mo, i, j, k, k) * 2.0 - ai(mo, i, k, j, k));
     } else {
         f = h + ai(mo, i, i, i, i);
@@ -96,27 +99,52 @@ __global__ void k_fock_elements(const double* __restrict__ mo, const double* __r
 }
 
 // fock.f:80-99: val, the closed-shell electronic energy (without the nuclear term), one
-// work-group.  The terms of row i are formed in parallel and then added by one lane in the
-// reference's order: 2 h_ii, (ii|ii), then tt(i,k) twice for each k < i.
+// work-group, after k_fock_elements.  The terms tt(i,k) = 2 (ii|kk) - (ik|ik) are formed
+// from the J and K rows just written (the same stored integrals ai() returns), staged in
+// LDS a block of rows at a time, and added by one lane in the reference's order:
+// 2 h_ii, (ii|ii), then tt(i,k) twice for each k < i.
 __global__ void __launch_bounds__(kEnergyThreads)
-k_fock_energy(const double* __restrict__ mo, const double* __restrict__ hmo, int noc,
-              double* __restrict__ energy) {
+k_fock_energy(const double* __restrict__ hmo, const double* __restrict__ jint,
+              const double* __restrict__ kint, int noc, double* __restrict__ energy) {
 #pragma clang fp contract(off)
-    __shared__ double tt[kMaxOcc];
+    __shared__ double tt[kEnergyStage];
+    __shared__ double hd[kMaxMo];
     double val = 0.0;
-    for (int i = 0; i < noc; ++i) {
-        for (int k = threadIdx.x; k < i; k += blockDim.x) {
-            double t = ai(mo, i, i, k, k);
-            const double tk = ai(mo, i, k, i, k);
-            tt[k] = t + t - tk;
+    int i0 = 0;
+    while (i0 < noc) {
+        const int64_t base = pair0(i0, 0);
+        int i1 = i0 + 1;
+        while (i1 < noc && pair0(i1 + 1, 0) - base <= kEnergyStage) ++i1;
+        const int cnt = (int)(pair0(i1, 0) - base);      // rows i0 .. i1-1, k <= i
+        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
+            const double jv = jint[base + x], kv = kint[base + x];
+            tt[x] = jv + jv - kv;                          // overwritten below on the diagonal
+        }
+        __syncthreads();
+        for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+            const int64_t d = pair0(i, i);
+            tt[d - base] = jint[d];                        // (ii|ii)
+            hd[i - i0] = hmo[d];
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            val = val + hmo[pair0(i, i)] * 2.0;
-            val = val + ai(mo, i, i, i, i);
-            for (int k = 0; k < i; ++k) val = val + tt[k] + tt[k];
+            for (int i = i0; i < i1; ++i) {
+                const int r = (int)(pair0(i, 0) - base);
+                val = val + hd[i - i0] * 2.0;
+                val = val + tt[r + i];
+                int k = 0;
+                for (; k + 8 <= i; k += 8) {
+                    double t[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) t[u] = tt[r + k + u];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) val = val + t[u] + t[u];
+                }
+                for (; k < i; ++k) val = val + tt[r + k] + tt[r + k];
+            }
         }
         __syncthreads();
+        i0 = i1;
     }
     if (threadIdx.x == 0) *energy = val;
 }
@@ -162,14 +190,14 @@ int fock_impl(cipsi_ijkl_ctx* ctx, int m, int noc, const double* d_mo, const dou
     const int64_t pm = pair0(m, 0);
     hipLaunchKernelGGL(k_fock_elements, dim3((unsigned)((pm + 63) / 64)), dim3(64), 0, ctx->stream,
                        d_mo, d_hmo, m, noc, d_fock, d_j, d_k);
-    hipLaunchKernelGGL(k_fock_energy, dim3(1), dim3(kEnergyThreads), 0, ctx->stream, d_mo, d_hmo,
-                       noc, d_energy);
+    hipLaunchKernelGGL(k_fock_energy, dim3(1), dim3(kEnergyThreads), 0, ctx->stream, d_hmo, d_j,
+                       d_k, noc, d_energy);
     return hipGetLastError() == hipSuccess ? CIPSI_OK : CIPSI_ERR_HIP;
 }
 
 int check_fock_args(int m, int noc) {
     if (m < 1 || noc < 0 || noc > m) return CIPSI_ERR_ARG;
-    if (m > kMaxOcc) return CIPSI_ERR_UNSUPPORTED;
+    if (m > kMaxMo) return CIPSI_ERR_UNSUPPORTED;
     return CIPSI_OK;
 }
 
