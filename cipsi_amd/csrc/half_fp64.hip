@@ -144,35 +144,55 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
 #pragma unroll
     for (int i = 0; i < NRB; ++i) acc[i] = f64x4{0.0, 0.0, 0.0, 0.0};
 
+    // B operand C(s0 + k4 + lg, cw + lm) (C is zero-padded to kCRows rows, so every
+    // chunk's rows exist).  A chunk's KC/4 values are loaded during the previous chunk,
+    // before that chunk's Z prefetch: loads complete in order, so the MFMAs of a chunk
+    // wait only for C loads issued a whole chunk earlier, never for the HBM prefetch of
+    // the next Z panel.
+    constexpr int NB = KC / 4;
+    gcdouble* __restrict__ cbw = cr + (int64_t)(cw >> 4) * (kCRows * 16) + lg * 16 + lm;
+    double bcur[NB];
     const int nchunk = (np + KC - 1) / KC;
+    if (wave_active) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) bcur[k] = cbw[(4 * k) * 16];
+    }
     fetch(0);
     commit(0, 0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0), as at the end of a chunk
     __syncthreads();
     for (int ch = 0; ch < nchunk; ++ch) {
         const bool more = (ch + 1) < nchunk;
+        double bnext[NB];
+        if (more && wave_active) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) bnext[k] = cbw[((ch + 1) * KC + 4 * k) * 16];
+        }
+        __builtin_amdgcn_sched_barrier(0);
         if (more) fetch((ch + 1) * KC);
+        __builtin_amdgcn_sched_barrier(0);
         if (wave_active) {
             const int zo = (ch & 1) * PANEL;
-            const int s0 = ch * KC;
-            // B operand C(s0 + k4 + lg, cw + lm) (C is zero-padded to kCRows rows),
-            // loaded one 4-deep K step ahead
-            gcdouble* __restrict__ cbw = cr + (int64_t)(cw >> 4) * (kCRows * 16);
-            const int boff = (s0 + lg) * 16 + lm;
-            double bcur = cbw[boff];
 #pragma unroll
-            for (int k4 = 0; k4 < KC; k4 += 4) {
-                const double bnext = k4 + 4 < KC ? cbw[boff + (k4 + 4) * 16] : 0.0;
+            for (int k = 0; k < NB; ++k) {
 #pragma unroll
                 for (int rb = 0; rb < NRB; ++rb) {
-                    const double af = smem[zo + (k4 + lg) * LDZ + rb * 16 + lm];
-                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bcur, acc[rb], 0, 0, 0);
+                    const double af = smem[zo + (4 * k + lg) * LDZ + rb * 16 + lm];
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bcur[k], acc[rb], 0, 0, 0);
                 }
-                bcur = bnext;
             }
         }
         __builtin_amdgcn_sched_barrier(0);
         if (more) commit((ch + 1) * KC, ((ch + 1) & 1) * PANEL);
+        // vmcnt(0): every load of this chunk has been consumed on the paths that use it;
+        // without the explicit wait the compiler keeps the skipped commit slots pending
+        // into the next chunk and stalls its prefetch on them.
+        __builtin_amdgcn_s_waitcnt(0x0F70);
         __syncthreads();
+        if (more && wave_active) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) bcur[k] = bnext[k];
+        }
     }
     if (!wave_active) return;
 
@@ -184,23 +204,26 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     const int nab = (c1 - 1) / 16 + 1;
     const int my_last_ab = min(nab - 1, cw / 16);
     gdouble* __restrict__ dst = (gdouble*)A.dst + dst_off;
+    // C rows of row block rb of column block ab (zero-padded to kCRows rows), prefetched
+    // QD row blocks ahead through a ring of QD + 1.  The first QD row blocks of the next
+    // column block are loaded before this block's results are stored: stores count in
+    // the same in-order counter as loads, so loads issued after a store could only be
+    // waited for together with it.
+    const int qoff = lg * 16 + lm;
+    constexpr int QD = KC == 16 ? 3 : 1;               // registers allow a deeper ring at KC 16
+    double ring[QD + 1][4];
+    auto load_rb = [&](int ab, int rb, double* bq) {
+        gcdouble* __restrict__ cba = cr + (int64_t)ab * (kCRows * 16);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bq[q] = cba[qoff + (16 * rb + 4 * q) * 16];
+    };
+#pragma unroll
+    for (int rb = 0; rb < QD && rb < NRB; ++rb) load_rb(0, rb, ring[rb]);
     for (int ab = 0; ab <= my_last_ab; ++ab) {
         f64x4 o0 = f64x4{0.0, 0.0, 0.0, 0.0}, o1 = o0;
-        // C rows of row block rb (zero-padded to kCRows rows), prefetched QD blocks
-        // ahead through a ring of QD + 1
-        gcdouble* __restrict__ cba = cr + (int64_t)ab * (kCRows * 16);
-        const int qoff = lg * 16 + lm;
-        constexpr int QD = KC == 16 ? 3 : 1;           // registers allow a deeper ring at KC 16
-        double ring[QD + 1][4];
-        auto load_rb = [&](int rb, double* bq) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) bq[q] = cba[qoff + (16 * rb + 4 * q) * 16];
-        };
-#pragma unroll
-        for (int rb = 0; rb < QD && rb < NRB; ++rb) load_rb(rb, ring[rb]);
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb) {
-            if (rb + QD < NRB) load_rb(rb + QD, ring[(rb + QD) % (QD + 1)]);
+            if (rb + QD < NRB) load_rb(ab, rb + QD, ring[(rb + QD) % (QD + 1)]);
             const double* bq = ring[rb % (QD + 1)];
             o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][0], bq[0], o0, 0, 0, 0);
             o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][1], bq[1], o1, 0, 0, 0);
@@ -208,6 +231,11 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
             o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][3], bq[3], o1, 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
+        if (ab < my_last_ab) {
+#pragma unroll
+            for (int rb = 0; rb < QD && rb < NRB; ++rb) load_rb(ab + 1, rb, ring[rb]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
         // lane (lm, lg), element q: O(a = 16 ab + lm, c = cw + lg + 4q)
         const int a = ab * 16 + lm;
 #pragma unroll
