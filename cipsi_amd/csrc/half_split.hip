@@ -208,32 +208,41 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
     constexpr int EPT = TR * KCH / NTHR;                            // 2
     double xs[EPT];
     gcdouble* __restrict__ src = (gcdouble*)A.src;
+    // Loads are unconditional (an element outside the matrix reads element 0 of its row,
+    // which exists) and masked at commit, so no register is written by anything but its
+    // load and the loop body has no branches around loads.
+    auto elem = [&](int u, int ch, int64_t& idx) -> bool {
+        const int e = tid + u * NTHR;
+        const int i = e / KCH, k = e % KCH;
+        const int r = row0 + i, s = ch * KCH + k;
+        bool ok;
+        if (STEP == 1) {
+            ok = r < A.n && s < A.n;
+            const int hi = r >= s ? r : s, lo = r >= s ? s : r;
+            idx = b * A.src_ld + (ok ? pair0(hi, lo) : 0);
+        } else {
+            ok = r < nrows && s < A.n;
+            idx = ok ? (b * A.mp + r) * (int64_t)A.kp + s : b * A.mp * (int64_t)A.kp;
+        }
+        return ok;
+    };
     auto fetch = [&](int ch) {
 #pragma unroll
         for (int u = 0; u < EPT; ++u) {
-            const int e = tid + u * NTHR;
-            const int i = e / KCH, k = e % KCH;
-            const int r = row0 + i, s = ch * KCH + k;
-            int64_t idx = 0;
-            bool ok;
-            if (STEP == 1) {
-                ok = r < A.n && s < A.n;
-                const int hi = r >= s ? r : s, lo = r >= s ? s : r;
-                if (ok) idx = b * A.src_ld + pair0(hi, lo);
-            } else {
-                ok = r < nrows && s < A.n;
-                if (ok) idx = (b * A.mp + r) * (int64_t)A.kp + s;
-            }
-            xs[u] = ok ? src[idx] : 0.0;
+            int64_t idx;
+            (void)elem(u, ch, idx);
+            xs[u] = src[idx];
         }
     };
-    auto commit = [&](int buf) {
+    auto commit = [&](int ch, int buf) {
 #pragma unroll
         for (int u = 0; u < EPT; ++u) {
             const int e = tid + u * NTHR;
             const int i = e / KCH, k = e % KCH;
+            int64_t idx;
+            const bool ok = elem(u, ch, idx);
             _Float16 d[S];
-            digits<S>(xs[u], rx[i], d);
+            digits<S>(ok ? xs[u] : 0.0, rx[i], d);
 #pragma unroll
             for (int s = 0; s < S; ++s) As[buf][s][i][k] = d[s];
         }
@@ -250,43 +259,62 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
     typedef __attribute__((address_space(1))) const f16x8 gcf16x8;
     gcf16x8* __restrict__ csv = (gcf16x8*)A.cs;
 
-    fetch(0);
-    commit(0);
-    __syncthreads();
-    for (int ch = 0; ch < kc; ++ch) {
-        const bool more = ch + 1 < kc;
-        if (more) fetch(ch + 1);
-        if (active) {
-            const int buf = ch & 1;
-            f16x8 af[S][2];
+    // B fragments (digits of C), double-buffered in registers: the next chunk's S x 2
+    // fragments are loaded at the start of a chunk, ahead of its A prefetch, by every
+    // wave (a wave without columns reads a clamped column), so that no load sits in a
+    // branch.  Loads complete in order: the commit waits only for the A prefetch, and a
+    // chunk's MFMAs wait only for B loads issued a whole chunk earlier.  Columns past mp
+    // only feed outputs that are never stored.  The A fragments are read from LDS per
+    // digit i, so only one pair is live.
+    f16x8 bcur[S][2], bnext[S][2];
+    auto load_b = [&](int ch, f16x8 (&bf)[S][2]) {
 #pragma unroll
-            for (int s = 0; s < S; ++s)
+        for (int j = 0; j < S; ++j)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                const int c = min(cw + 16 * cb + lm, A.mp - 1);
+                bf[j][cb] = csv[(((int64_t)j * kc + ch) * A.mp + c) * (KCH / 8) + lg];
+            }
+    };
+    auto mfma_chunk = [&](int ch) {
+        const int buf = ch & 1;
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            f16x8 af[2];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+                af[rb] = *(const f16x8*)&As[buf][i][16 * rb + lm][8 * lg];
+#pragma unroll
+            for (int j = 0; i + j < S; ++j)
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
-                    af[s][rb] = *(const f16x8*)&As[buf][s][16 * rb + lm][8 * lg];
 #pragma unroll
-            for (int j = 0; j < S; ++j) {
-                f16x8 bf[2];
-#pragma unroll
-                for (int cb = 0; cb < 2; ++cb) {
-                    // columns past mp only feed outputs that are never stored
-                    const int c = min(cw + 16 * cb + lm, A.mp - 1);
-                    bf[cb] = csv[(((int64_t)j * kc + ch) * A.mp + c) * (KCH / 8) + lg];
-                }
-#pragma unroll
-                for (int i = 0; i + j < S; ++i)
-#pragma unroll
-                    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                        for (int cb = 0; cb < 2; ++cb)
-                            acc[i + j][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                                af[i][rb], bf[cb], acc[i + j][rb][cb], 0, 0, 0);
-            }
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc[i + j][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                            af[rb], bcur[j][cb], acc[i + j][rb][cb], 0, 0, 0);
         }
+    };
+    load_b(0, bcur);
+    fetch(0);
+    commit(0, 0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0): nothing pending into the loop
+    __syncthreads();
+    // all chunks but the last prefetch the next chunk's B and A; the last chunk is peeled
+    // so that the loop body has no conditional loads
+    for (int ch = 0; ch + 1 < kc; ++ch) {
+        load_b(ch + 1, bnext);
+        fetch(ch + 1);
         __builtin_amdgcn_sched_barrier(0);
-        if (more) commit((ch + 1) & 1);
+        if (active) mfma_chunk(ch);
+        __builtin_amdgcn_sched_barrier(0);
+        commit(ch + 1, (ch + 1) & 1);
         __syncthreads();
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) bcur[j][cb] = bnext[j][cb];
     }
+    if (active) mfma_chunk(kc - 1);
     if (!active) return;
 
     // ---- epilogue: FP64 recombination and stores
