@@ -38,6 +38,9 @@ typedef __attribute__((address_space(1))) const double gcdouble;
 typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) const int64_t gcint64;
 
+// The value the Q1 staging loads for panel elements outside the matrix.
+__device__ double kZeroElement[1] = {0.0};
+
 __device__ inline void work_item(const HalfArgs& A, int lt, int64_t item, int64_t& b, int& tile,
                                  int& ncols, int64_t& limit, int64_t& dst_off) {
     if (A.half == 1) {
@@ -103,7 +106,8 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     // Panel = Z(r, s0 + kk) for r < np, kk < KC, taken in 16-row passes.  Below the
     // diagonal block (r >= s) a row's KC entries are contiguous in the packed row
     // (kk fastest across lanes); above it (r < s0) the entries of one s are contiguous
-    // in r (r fastest across lanes).  Branch-free: invalid elements load index 0.
+    // in r (r fastest across lanes).  Branch-free: an element outside the matrix is
+    // loaded from a zero in global memory, so the commit stores what arrived.
     double zst[K::NU];
     auto zslot = [&](int u, int& p, int& e) {
         const int sweep = u / K::SUB, sub = u % K::SUB;
@@ -112,8 +116,9 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
         e = t % K::EP;
     };
     auto zmap = [&](int p, int e, int s0, int& r, int& kk) {
-        if (16 * p + 15 < s0) { r = 16 * p + (e & 15); kk = e >> 4; }
-        else { r = 16 * p + e / KC; kk = e % KC; }
+        const bool above = 16 * p + 15 < s0;
+        r = 16 * p + (above ? (e & 15) : e / KC);
+        kk = above ? (e >> 4) : (e % KC);
     };
     auto fetch = [&](int s0) {
 #pragma unroll
@@ -122,10 +127,10 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
             zslot(u, p, e);
             zmap(p, e, s0, r, kk);
             const int s = s0 + kk;
-            const bool ok = (r < n) && (s < n);
-            const int hi = r >= s ? r : s, lo = r >= s ? s : r;
-            const int idx = ok ? ((hi * (hi + 1)) >> 1) + lo : 0;
-            zst[u] = zsrc[idx];            // masked at commit: no wait on the load here
+            const unsigned hi = r >= s ? r : s, lo = r >= s ? s : r;
+            const unsigned idx = ((hi * (hi + 1)) >> 1) + lo;
+            gcdouble* ptr = (r < n && s < n) ? zsrc + idx : (gcdouble*)kZeroElement;
+            zst[u] = *ptr;                 // waited for at commit, not here
         }
     };
     auto commit = [&](int s0, int boff) {
@@ -134,8 +139,7 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
             int p, e, r, kk;
             zslot(u, p, e);
             zmap(p, e, s0, r, kk);
-            const bool ok = (r < n) && (s0 + kk < n);
-            if (p < NRB) smem[boff + kk * LDZ + r] = ok ? zst[u] : 0.0;
+            if (p < NRB) smem[boff + kk * LDZ + r] = zst[u];
         }
     };
 
