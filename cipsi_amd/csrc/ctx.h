@@ -39,7 +39,7 @@ struct cipsi_ijkl_ctx {
     int64_t ts_total = 0;
     // tile plans (kernel variant index) for the two halves: defaults and per-size plans
     int var_half1 = 2;
-    int var_half2 = 5;
+    int var_half2 = 2;
     std::map<std::tuple<int, int, int>, std::pair<int, int>> plans;
     // per-stage event timing
     bool timing = false;

@@ -1,8 +1,10 @@
 // FP64 MFMA half-transform ("sandwich") kernel for gfx950.
 //
 // One work-group = one symmetric matrix Z_b (an AO pair for half 1, an MO pair for
-// half 2) times one tile of NW*16 output columns; wave w owns the 16 columns
-// c_w = c0 + 16w .. c_w + 15 and every row block:
+// half 2) times one tile of up to NW 16-column blocks; the nt tiles of a matrix take
+// the column blocks round robin (tile t: blocks t, t + nt, t + 2nt, ...), so every tile
+// has work for about the same number of waves and a similar share of the triangle.
+// Wave w owns one block, 16 columns c_w .. c_w + 15, and every row block:
 //   Q1/Q3:  T(r, c) = sum_s Z_b(r, s) C(s, c)          (tpqkld.f:82-90 / tijkld.f:53-60)
 //   Q2/Q4:  O(a, c) = sum_r C(r, a) T(r, c), a <= c    (tpqkld.f:91-97 / tijkld.f:61-76)
 //
@@ -89,13 +91,13 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     int64_t b, limit, dst_off;
     int tile, ncols;
     work_item(A, K::LT, item, b, tile, ncols, limit, dst_off);
-    const int c0 = tile * K::LT;
-    const int c1 = min(c0 + K::LT, ncols);
-    // Column block of this wave.  Waves w and w + 4 share a SIMD; with 8 waves they
-    // take blocks s and 7 - s, so that every SIMD gets the same share of the Q2
-    // triangle.
+    const int ntiles = (ncols + K::LT - 1) / K::LT;
+    const int c1 = ncols;
+    // Column block of this wave: slot wc of the tile.  Waves w and w + 4 share a SIMD;
+    // with 8 waves they take slots s and 7 - s, blocks t + s nt and t + (7 - s) nt, so
+    // that every SIMD gets the same share of the Q2 triangle.
     const int wc = NW == 8 ? (w < 4 ? w : 11 - w) : w;
-    const int cw = c0 + 16 * wc;                   // first column of this wave
+    const int cw = 16 * (tile + ntiles * wc);     // first column of this wave
     const bool wave_active = cw < c1;
     gcdouble* __restrict__ zsrc = (gcdouble*)A.src + b * A.src_ld;
     gcdouble* __restrict__ cr = (gcdouble*)A.cr;
@@ -205,8 +207,7 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     // T(16 rb + 4 q + lg, cw + lm) in acc[rb][q] (the A operand) and loads
     // C(16 rb + 4 q + lg, 16 ab + lm) as the B operand.  Wave w needs a <= its last
     // column, so it stops after block cw / 16.
-    const int nab = (c1 - 1) / 16 + 1;
-    const int my_last_ab = min(nab - 1, cw / 16);
+    const int my_last_ab = cw / 16;
     gdouble* __restrict__ dst = (gdouble*)A.dst + dst_off;
     // C rows of row block rb of column block ab (zero-padded to kCRows rows), prefetched
     // QD row blocks ahead through a ring of QD + 1.  The first QD row blocks of the next
