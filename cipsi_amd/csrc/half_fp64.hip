@@ -93,10 +93,21 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     work_item(A, K::LT, item, b, tile, ncols, limit, dst_off);
     const int ntiles = (ncols + K::LT - 1) / K::LT;
     const int c1 = ncols;
-    // Column block of this wave: slot wc of the tile.  Waves w and w + 4 share a SIMD;
-    // with 8 waves they take slots s and 7 - s, blocks t + s nt and t + (7 - s) nt, so
-    // that every SIMD gets the same share of the Q2 triangle.
-    const int wc = NW == 8 ? (w < 4 ? w : 11 - w) : w;
+    // Column block of this wave: slot wc of the tile, block tile + ntiles * wc.  The
+    // tile has nact busy slots, and slot i's Q2 work grows with i.  Waves w and w + 4
+    // share SIMD w % 4, so with 8 waves P = max(0, nact - 4) SIMDs must hold two busy
+    // waves: SIMD j < P takes slots j and 2P - 1 - j (smallest with largest), the other
+    // SIMDs take the largest slots 2P .. nact - 1 alone.  This evens out both the Q1
+    // waves and the Q2 triangle per SIMD.  Slots >= nact are idle.
+    int wc = w;
+    if (NW == 8) {
+        const int nblk = (ncols + 15) >> 4;
+        const int nact = (nblk - tile + ntiles - 1) / ntiles;
+        const int p2 = nact > 4 ? nact - 4 : 0;
+        const int j = w & 3;
+        if (j < p2) wc = (w >> 2) ? 2 * p2 - 1 - j : j;
+        else wc = (w >> 2) ? nact + j : p2 + j;
+    }
     const int cw = 16 * (tile + ntiles * wc);     // first column of this wave
     const bool wave_active = cw < c1;
     gcdouble* __restrict__ zsrc = (gcdouble*)A.src + b * A.src_ld;

@@ -125,3 +125,29 @@ def test_errors_reported_not_fatal(ctx):
     # the context is still usable afterwards
     out = ctx.transform(V, c)
     assert np.max(np.abs(out - oracle.ijkl(V, c))) <= TOL
+
+
+@pytest.mark.parametrize("n,m", [(40, 24), (97, 90), (200, 177)])
+def test_every_tile_variant_gives_the_same_bits(ctx, n, m):
+    """Every tile variant pair (waves, panel depth, occupancy hint) sums each element in
+    the same order, so the packed output is bit-identical across plans; the tiles deal
+    the column blocks of a matrix round robin, and ragged half-2 rows give tiles with
+    every number of busy waves.  The smallest size is also checked against the oracle."""
+    from cipsi_amd import ijkl
+    print(f"[variants] n={n} m={m}", flush=True)
+    case = synth.make_case(n, m, 2 * n, 6.0, 300 + n)
+    V, c = dev(case["ao_packed"]), dev(case["c"])
+    ref = ctx.transform(V, c)
+    torch.cuda.synchronize()
+    if n <= 40:
+        assert np.max(np.abs(ref.cpu().numpy() - oracle.ijkl(case["ao_packed"], case["c"]))) <= TOL
+    nv = len(ijkl.variants())
+    try:
+        for v in range(nv):
+            for v2 in (v, (v + 3) % nv):
+                ctx.set_plan(n, m, v, v2)
+                out = ctx.transform(V, c)
+                torch.cuda.synchronize()
+                assert torch.equal(out, ref), f"variants ({v}, {v2})"
+    finally:
+        ctx.set_plan(n, m, *ctx.get_plan(1, 1))    # back to the context defaults
