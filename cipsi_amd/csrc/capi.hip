@@ -319,13 +319,12 @@ int cipsi_ijkl_unpack_labels_dev(cipsi_ijkl_ctx* ctx, int n_ao, const double* d_
     const int64_t nblk = npairs(npairs(n_ao));
     if ((rc = ctx_alloc(ctx, ctx->owner, sizeof(uint32_t) * nblk))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->scal, 64))) return rc;
-    int64_t* first_zero = (int64_t*)ctx->scal.ptr;
-    int* err_flag = (int*)((char*)ctx->scal.ptr + 16);
+    unsigned long long* sc = (unsigned long long*)ctx->scal.ptr;   // 3 scalars
+    int* err_flag = (int*)((char*)ctx->scal.ptr + 32);
     {
         StageTimer tm(ctx, "unpack");
         HIP_TRY(ctx, launch_unpack_labels(d_xx, d_labels, n_entries, n_ao, d_ao,
-                                          (uint32_t*)ctx->owner.ptr, first_zero, err_flag,
-                                          ctx->stream));
+                                          (uint32_t*)ctx->owner.ptr, sc, err_flag, ctx->stream));
     }
     int herr = 0;
     HIP_TRY(ctx, hipMemcpyAsync(&herr, err_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

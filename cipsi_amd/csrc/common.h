@@ -76,8 +76,8 @@ inline int variant_columns(int v) { return 16 * kHalfVariants[v].waves; }
 
 // Device launchers (defined in the .hip files).
 hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t n_entries,
-                                int n_ao, double* ao_packed, uint32_t* owner, int64_t* first_zero,
-                                int* err_flag, hipStream_t st);
+                                int n_ao, double* ao_packed, uint32_t* owner,
+                                unsigned long long* sc, int* err_flag, hipStream_t st);
 hipError_t launch_expand_packed(const double* ao_packed, int64_t pa, double* vsq, hipStream_t st);
 hipError_t launch_transpose(const double* src, int64_t rows, int64_t cols, double* dst,
                             hipStream_t st);

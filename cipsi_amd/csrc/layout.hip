@@ -36,87 +36,189 @@ hipError_t launch_zero(double* p, int64_t n, hipStream_t st) {
 // ----------------------------------------------------------------------------- labels
 // transd.f:85-97 (decode, first zero label ends the stream), 210-218 (doubling of
 // the i==j, k==l and ij==kl entries that pshf/qout.f:73-77 halved), 369-372 (store
-// at ij1(ij1-1)/2 + kl1).  The reference processes labels sequentially, so a later
-// duplicate overwrites an earlier one: pass 1 elects the last writer per position
-// with atomicMax on (index+1), pass 2 lets only that writer store.
+// at ij1(ij1-1)/2 + kl1), into a block zeroed first (itijkl.f:191-192).  The reference
+// processes labels sequentially, so a later duplicate overwrites an earlier one: the
+// last writer of each position is elected with atomicMax on (index + 1).
+//
+// Passes (all streaming in the order pshf writes the labels):
+//   claim   every label of the buffer: a zero label lowers the stream end, a label with
+//           a field outside 1..n_ao lowers the first bad index, a valid one claims its
+//           position and raises the last valid index;
+//   fix     only when a valid label lies after the first zero (the stale tail of the
+//           last record): the owners are cleared and re-claimed up to the stream end;
+//   resolve one thread per position: the owner's value times 2^(doubling), or 0.
+// Scalars sc[]: 0 = stream end (first zero label), 1 = first bad label, 2 = last valid
+// label + 1 (0 = none), all as unsigned 64-bit for the atomics.
 
-__global__ void k_init_first_zero(int64_t* first_zero, int64_t n, int* err_flag) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) { *first_zero = n; *err_flag = 0; }
-}
-
-__global__ void k_find_first_zero(const int32_t* __restrict__ labels, int64_t n,
-                                  int64_t* first_zero) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride)
-        if (labels[i] == 0) atomicMin((unsigned long long*)first_zero, (unsigned long long)i);
-}
-
-__device__ inline bool decode_label(int32_t lab, int n_ao, int64_t nblk, int64_t* pos,
-                                    int* dbl) {
-    uint32_t u = (uint32_t)lab;
-    int i = (int)((u >> 24) & 255u), j = (int)((u >> 16) & 255u);
-    int k = (int)((u >> 8) & 255u), l = (int)(u & 255u);
-    if (i < 1 || j < 1 || k < 1 || l < 1 || i > n_ao || j > n_ao || k > n_ao || l > n_ao)
-        return false;
-    int64_t ij1 = (int64_t)i * (i - 1) / 2 + j;
-    int64_t kl1 = (int64_t)k * (k - 1) / 2 + l;
-    int64_t p = ij1 * (ij1 - 1) / 2 + kl1 - 1;          // 0-based
-    if (p < 0 || p >= nblk) return false;
-    *pos = p;
-    *dbl = (i == j) + (k == l) + (i == k && j == l);
-    return true;
-}
-
-__global__ void k_claim_labels(const int32_t* __restrict__ labels, const int64_t* first_zero,
-                               int n_ao, int64_t nblk, uint32_t* __restrict__ owner,
-                               int* err_flag) {
-    const int64_t n = *first_zero;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) {
-        int64_t pos;
-        int dbl;
-        if (!decode_label(labels[i], n_ao, nblk, &pos, &dbl)) { atomicOr(err_flag, 1); continue; }
-        atomicMax(owner + pos, (uint32_t)(i + 1));
+__global__ void k_init_label_scalars(unsigned long long* sc, int64_t n, int* err_flag) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc[0] = (unsigned long long)n;
+        sc[1] = (unsigned long long)n;
+        sc[2] = 0ull;
+        *err_flag = 0;
     }
 }
 
-__global__ void k_store_labels(const double* __restrict__ xx, const int32_t* __restrict__ labels,
-                               const int64_t* first_zero, int n_ao, int64_t nblk,
-                               const uint32_t* __restrict__ owner, double* __restrict__ out) {
-    const int64_t n = *first_zero;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) {
+// 1-based fields (transd.f:91-97) -> 0-based packed position; false if out of range.
+__device__ inline bool decode_label(int32_t lab, int n_ao, int64_t* pos) {
+    const uint32_t u = (uint32_t)lab;
+    const int i = (int)((u >> 24) & 255u), j = (int)((u >> 16) & 255u);
+    const int k = (int)((u >> 8) & 255u), l = (int)(u & 255u);
+    if (i < 1 || j < 1 || k < 1 || l < 1 || i > n_ao || j > n_ao || k > n_ao || l > n_ao)
+        return false;
+    const int64_t ij1 = (int64_t)i * (i - 1) / 2 + j;
+    const int64_t kl1 = (int64_t)k * (k - 1) / 2 + l;
+    const int64_t p = ij1 * (ij1 - 1) / 2 + kl1 - 1;   // 0-based
+    if (p < 0 || p >= npairs(npairs(n_ao))) return false;
+    *pos = p;
+    return true;
+}
+
+constexpr int kLabelThreads = 256;
+
+// Block-wide min / max of per-thread values, then one atomic per block.
+__device__ inline void block_commit(unsigned long long zmin, unsigned long long bmin,
+                                    unsigned long long vmax, unsigned long long* sc) {
+    __shared__ unsigned long long r[3][kLabelThreads / 64];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        zmin = min(zmin, (unsigned long long)__shfl_xor(zmin, o));
+        bmin = min(bmin, (unsigned long long)__shfl_xor(bmin, o));
+        vmax = max(vmax, (unsigned long long)__shfl_xor(vmax, o));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { r[0][w] = zmin; r[1][w] = bmin; r[2][w] = vmax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < kLabelThreads / 64; ++i) {
+            zmin = min(zmin, r[0][i]);
+            bmin = min(bmin, r[1][i]);
+            vmax = max(vmax, r[2][i]);
+        }
+        if (zmin < sc[0]) atomicMin(&sc[0], zmin);
+        if (bmin < sc[1]) atomicMin(&sc[1], bmin);
+        if (vmax > sc[2]) atomicMax(&sc[2], vmax);
+    }
+}
+
+// claim over every label of the buffer.  Consecutive lanes take consecutive labels, so
+// the atomics of a wave instruction fall on as few cache lines as the stream allows.
+__global__ __launch_bounds__(kLabelThreads) void k_claim_scan(const int32_t* __restrict__ labels,
+                                                               int64_t n, int n_ao,
+                                                               uint32_t* __restrict__ owner,
+                                                               unsigned long long* sc) {
+    unsigned long long zmin = ~0ull, bmin = ~0ull, vmax = 0ull;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int32_t lab = labels[i];
         int64_t pos;
-        int dbl;
-        if (!decode_label(labels[i], n_ao, nblk, &pos, &dbl)) continue;
-        if (owner[pos] != (uint32_t)(i + 1)) continue;
-        double v = xx[i];
-        if (dbl >= 1) v = v + v;
-        if (dbl >= 2) v = v + v;
-        if (dbl >= 3) v = v + v;
-        out[pos] = v;
+        if (lab == 0) {
+            zmin = min(zmin, (unsigned long long)i);
+        } else if (!decode_label(lab, n_ao, &pos)) {
+            bmin = min(bmin, (unsigned long long)i);
+        } else {
+            atomicMax(owner + pos, (uint32_t)(i + 1));
+            vmax = max(vmax, (unsigned long long)(i + 1));
+        }
+    }
+    block_commit(zmin, bmin, vmax, sc);
+}
+
+// A valid label after the stream end claimed a position: clear and re-claim up to the end.
+__global__ void k_fix_clear(uint32_t* __restrict__ owner, int64_t nblk,
+                            const unsigned long long* sc) {
+    if (sc[2] <= sc[0] + 1) return;                  // last valid index < stream end
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nblk; p += stride)
+        owner[p] = 0u;
+}
+
+__global__ void k_fix_claim(const int32_t* __restrict__ labels, int n_ao,
+                            uint32_t* __restrict__ owner, const unsigned long long* sc) {
+    if (sc[2] <= sc[0] + 1) return;
+    const int64_t n = (int64_t)sc[0];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        int64_t pos;
+        if (decode_label(labels[i], n_ao, &pos)) atomicMax(owner + pos, (uint32_t)(i + 1));
+    }
+}
+
+// Largest r with r(r+1)/2 <= x, from a single-precision estimate made exact with
+// integer steps (x < 2^31 here: the packed block of N_AO <= 255 has 5.3e8 entries).
+__device__ inline int64_t tri_row(int64_t x) {
+    int64_t r = (int64_t)((sqrtf(8.0f * (float)x + 1.0f) - 1.0f) * 0.5f);
+    while (r * (r + 1) / 2 > x) --r;
+    while ((r + 1) * (r + 2) / 2 <= x) ++r;
+    return r;
+}
+
+// One thread per four consecutive positions: the first is decomposed into
+// (ab, cd) = pair indices and (i, j), (k, l) = AO indices, the next three by stepping.
+// The doubling count is (i == j) + (k == l) + (ab == cd) (transd.f:210-218).
+__global__ __launch_bounds__(kLabelThreads) void k_resolve(const double* __restrict__ xx,
+                                                            int64_t nblk,
+                                                            const uint32_t* __restrict__ owner,
+                                                            const unsigned long long* sc,
+                                                            int* err_flag,
+                                                            double* __restrict__ out) {
+    const uint64_t n = sc[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && sc[1] < n) *err_flag = 1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nq = (nblk + 3) >> 2;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+        const int64_t p0 = 4 * q;
+        const int64_t ab64 = tri_row(p0);
+        int ab = (int)ab64, cd = (int)(p0 - ab64 * (ab64 + 1) / 2);
+        int i = (int)tri_row(ab), k = (int)tri_row(cd);
+        int j = ab - i * (i + 1) / 2, l = cd - k * (k + 1) / 2;
+        uint32_t w[4];
+        if (p0 + 3 < nblk) {
+            const uint4 o = *(const uint4*)(owner + p0);         // owner is 16-byte aligned
+            w[0] = o.x; w[1] = o.y; w[2] = o.z; w[3] = o.w;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) w[e] = p0 + e < nblk ? owner[p0 + e] : 0u;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (p0 + e < nblk) {
+                double v = 0.0;
+                if (w[e] != 0u && (uint64_t)(w[e] - 1) < n) {
+                    v = xx[w[e] - 1];
+                    const int d = (i == j) + (k == l) + (ab == cd);
+                    if (d >= 1) v = v + v;
+                    if (d >= 2) v = v + v;
+                    if (d >= 3) v = v + v;
+                }
+                out[p0 + e] = v;
+            }
+            // step to the next position: cd, then (k, l); past the diagonal, the next ab
+            if (++l > k) { ++k; l = 0; }
+            if (++cd > ab) {
+                ++ab; cd = 0; k = 0; l = 0;
+                if (++j > i) { ++i; j = 0; }
+            }
+        }
     }
 }
 
 hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t n_entries,
-                                int n_ao, double* ao_packed, uint32_t* owner, int64_t* first_zero,
-                                int* err_flag, hipStream_t st) {
+                                int n_ao, double* ao_packed, uint32_t* owner,
+                                unsigned long long* sc, int* err_flag, hipStream_t st) {
     const int64_t pa = npairs(n_ao);
     const int64_t nblk = npairs(pa);
-    hipLaunchKernelGGL(k_init_first_zero, dim3(1), dim3(64), 0, st, first_zero, n_entries, err_flag);
-    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(nblk, 256)), dim3(256), 0, st, ao_packed, nblk, 0.0);
+    hipLaunchKernelGGL(k_init_label_scalars, dim3(1), dim3(64), 0, st, sc, n_entries, err_flag);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nblk, 256)), dim3(256), 0, st, owner, nblk, 0u);
     if (n_entries > 0) {
-        hipLaunchKernelGGL(k_find_first_zero, dim3(grid_for(n_entries, 256)), dim3(256), 0, st,
-                           labels, n_entries, first_zero);
-        hipLaunchKernelGGL(k_claim_labels, dim3(grid_for(n_entries, 256)), dim3(256), 0, st,
-                           labels, first_zero, n_ao, nblk, owner, err_flag);
-        hipLaunchKernelGGL(k_store_labels, dim3(grid_for(n_entries, 256)), dim3(256), 0, st,
-                           xx, labels, first_zero, n_ao, nblk, owner, ao_packed);
+        hipLaunchKernelGGL(k_claim_scan, dim3(grid_for(n_entries, kLabelThreads)),
+                           dim3(kLabelThreads), 0, st, labels, n_entries, n_ao, owner, sc);
+        hipLaunchKernelGGL(k_fix_clear, dim3(grid_for(nblk, 256)), dim3(256), 0, st, owner, nblk, sc);
+        hipLaunchKernelGGL(k_fix_claim, dim3(grid_for(n_entries, 256)), dim3(256), 0, st, labels,
+                           n_ao, owner, sc);
     }
+    hipLaunchKernelGGL(k_resolve, dim3(grid_for((nblk + 3) / 4, kLabelThreads)), dim3(kLabelThreads), 0, st,
+                       xx, nblk, owner, sc, err_flag, ao_packed);
     return hipGetLastError();
 }
 

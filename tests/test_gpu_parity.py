@@ -151,3 +151,41 @@ def test_every_tile_variant_gives_the_same_bits(ctx, n, m):
                 assert torch.equal(out, ref), f"variants ({v}, {v2})"
     finally:
         ctx.set_plan(n, m, *ctx.get_plan(1, 1))    # back to the context defaults
+
+
+def test_label_stream_duplicates_and_stale_tail(ctx):
+    """transd semantics on the device, bit for bit against the oracle (transd.f:85-97,
+    369-372): later duplicates overwrite earlier ones; the first zero label ends the
+    stream, and what follows it (the stale tail of the last record: valid labels and
+    labels outside the basis) is ignored; a label outside the basis before the end is an
+    error.  The 4-byte-offset views take the unaligned (scalar) path of the claim pass."""
+    from cipsi_amd import ijkl
+    n = 9
+    print("[labels-tail]", flush=True)
+    case = synth.make_case(n, n, 18, 3.0, 410)
+    xx, ix = formats.labels_from_packed(case["ao_packed"], n)
+    rng = np.random.default_rng(411)
+    perm = rng.permutation(xx.size)
+    xx, ix = xx[perm], ix[perm]
+    dup = rng.choice(xx.size, 40, replace=False)                  # later duplicates win
+    xx = np.concatenate([xx, rng.standard_normal(40)])
+    ix = np.concatenate([ix, ix[dup]])
+    stale = rng.choice(xx.size, 30, replace=False)                # valid labels after the end
+    tail_x = np.concatenate([rng.standard_normal(30), [1.0, 2.0]])
+    tail_i = np.concatenate([ix[stale], formats.label_encode([n + 1, 3], [1, 0], [1, 1], [1, 1])])
+    xx = np.concatenate([xx, [0.0], tail_x])
+    ix = np.concatenate([ix, [0], tail_i]).astype(np.int32)
+    want = oracle.transd(n, xx, ix)
+    for off in (0, 1):
+        x_d = dev(np.concatenate([[7.0] * off, xx]))[off:]
+        i_d = dev(np.concatenate([np.array([5] * off, np.int32), ix]))[off:]
+        got = ctx.transd(x_d, i_d, n)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(got.cpu().numpy(), want)
+    bad = ix.copy()
+    bad[3] = formats.label_encode(n + 1, 1, 1, 1)
+    with pytest.raises(ijkl.IjklError) as e:
+        ctx.transd(dev(xx), dev(bad), n)
+    assert e.value.status == -2
+    got = ctx.transd(dev(xx), dev(ix), n)                        # still usable
+    np.testing.assert_array_equal(got.cpu().numpy(), want)
