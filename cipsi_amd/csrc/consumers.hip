@@ -5,6 +5,9 @@
 //            pair indices, larger pair first; cip/ai.f:1-42 for the same lookup).
 //   fock     the closed-shell reference energy, the MO Fock operator and the J / K
 //            diagonals of CIPSI_sources/fock/fock.f:80-127.
+//   hdig     cip's determinant energies relative to the reference (cip/hdig.f:40-80),
+//   hntd     and its off-diagonal Slater-Condon elements (cip/hntd.f:36-168), for
+//            determinants held as excitations of the closed-shell reference.
 //
 // Conventions (0-based here, the reference is 1-based):
 //   mo[pair0(max(a,b), min(a,b))], a = pair(i,j), b = pair(k,l)   the packed MO stream
@@ -201,6 +204,215 @@ int check_fock_args(int m, int noc) {
     return CIPSI_OK;
 }
 
+
+// ----------------------------------------------------------------------------- Slater-Condon
+// cip's determinant arrays (common /spo/), all numbers 1-based as there: determinant d
+// has ne[d] excitations, holes trou[nd[d] + t] and particles part[nd[d] + t] (spin-
+// orbital numbers); spin-orbital s has orbital iorb[s - 1] and spin ispin[s - 1].
+struct DetArrays {
+    const int* ne;
+    const int* nd;
+    const int* trou;
+    const int* part;
+    const int* iorb;
+    const int* ispin;
+};
+constexpr int kMaxExc = 10;           // cip's ita(20) / itb(20) hold 2 x 10 entries
+
+// 0-based packed index of the 1-based pair (a, b) as cip's num(a) + b, not reordered.
+__device__ inline int64_t tri1(int64_t a, int64_t b) { return a * (a - 1) / 2 + b - 1; }
+__device__ inline int64_t tri1s(int64_t a, int64_t b) { return a >= b ? tri1(a, b) : tri1(b, a); }
+__device__ inline double ai1(const double* __restrict__ mo, int a, int b, int c, int e) {
+    return ai(mo, a - 1, b - 1, c - 1, e - 1);
+}
+
+// Every determinant and spin-orbital in range, at most kMaxExc excitations; counts the
+// violations.
+__global__ void k_sc_validate(DetArrays D, int n_det, int64_t n_exc, int n_so, int n_mo,
+                              unsigned long long* __restrict__ n_bad) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t s = t0; s < n_so; s += stride) {
+        const int o = D.iorb[s], sp = D.ispin[s];
+        if (o < 1 || o > n_mo || sp < 1 || sp > 2) atomicAdd(n_bad, 1ull);
+    }
+    for (int64_t d = t0; d < n_det; d += stride) {
+        const int e = D.ne[d], o = D.nd[d];
+        bool ok = e >= 0 && e <= kMaxExc && o >= 0 && (int64_t)o + e <= n_exc;
+        for (int t = 0; ok && t < e; ++t) {
+            const int h = D.trou[o + t], q = D.part[o + t];
+            ok = h >= 1 && h <= n_so && q >= 1 && q <= n_so;
+        }
+        if (!ok) atomicAdd(n_bad, 1ull);
+    }
+}
+
+// cip/hdig.f:40-80, fdiag(s) = F(iorb(s), iorb(s)) as cip/ijkf.f:33-36 sets it.
+__global__ void k_hdig(DetArrays D, int n_det, const double* __restrict__ fock,
+                       const double* __restrict__ jint, const double* __restrict__ kint,
+                       double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n_det) return;
+    const int nei = D.ne[d];
+    double ab = 0.0;
+    const int l1 = D.nd[d];
+    for (int l = l1; l < l1 + nei; ++l) {
+        const int s1p = D.part[l], s1t = D.trou[l];
+        const int n1p = D.iorb[s1p - 1], n1t = D.iorb[s1t - 1];
+        ab = ab + fock[tri1(n1p, n1p)] - fock[tri1(n1t, n1t)];
+        const int ns1t = D.ispin[s1t - 1], ns1p = D.ispin[s1p - 1];
+        for (int j = l1; j <= l; ++j) {
+            const int s2t = D.trou[j], s2p = D.part[j];
+            const int ns2t = D.ispin[s2t - 1], ns2p = D.ispin[s2p - 1];
+            const int n2t = D.iorb[s2t - 1], n2p = D.iorb[s2p - 1];
+            const int64_t n12t = tri1s(n1t, n2t), n12p = tri1s(n1p, n2p);
+            const int64_t n1tp = tri1(n2p, n1t), n2tp = tri1(n1p, n2t);
+            double ca = jint[n12t] + jint[n12p] - jint[n1tp] - jint[n2tp];
+            if (ns1t == ns2t) ca = ca - kint[n12t];
+            if (ns1p == ns2p) ca = ca - kint[n12p];
+            if (ns1t == ns2p) ca = ca + kint[n1tp];
+            if (ns1p == ns2t) ca = ca + kint[n2tp];
+            if (j == l) ca = 0.5 * ca;
+            ab = ab + ca;
+        }
+    }
+    out[d] = ab;
+}
+
+// cip/hntd.f:36-168 for one pair (da, db): the determinant with more excitations is
+// "1"; its excitation list is reordered to line up with the other's (each swap flips the
+// sign); unmatched spin-orbitals give nv1 -> nv2; one difference: F(n1, n2) corrected
+// for the holes and particles of determinant 1; two: (n1 n2|n3 n4) - (n1 n4|n3 n2) by
+// spin; identical determinants or more than two differences: 0.
+__device__ double hntd_pair(const DetArrays& D, int da, int db, int norb,
+                            const double* __restrict__ fock, const double* __restrict__ mo) {
+#pragma clang fp contract(off)
+    int ita[2 * kMaxExc], itb[2 * kMaxExc];
+    int nv1[2] = {0, 0}, nv2[2] = {0, 0}, natu1[2] = {0, 0}, natu2[2] = {0, 0};
+    const int kr = (D.ne[da] - D.ne[db] <= 0) ? db : da;
+    const int kl = (kr == db) ? da : db;
+    const int ne1 = D.ne[kr];
+    if (ne1 == 0) return 0.0;
+    const int ne2 = D.ne[kl];
+    int nbdif = ne1 - ne2;
+    if (nbdif > 2) return 0.0;
+    for (int t = 0; t < ne1; ++t) {
+        ita[t] = D.trou[D.nd[kr] + t];
+        ita[t + ne1] = D.part[D.nd[kr] + t];
+    }
+    bool ysig = false;
+    if (ne2 > 0) {
+        for (int t = 0; t < ne2; ++t) {
+            itb[t] = D.trou[D.nd[kl] + t];
+            itb[t + ne2] = D.part[D.nd[kl] + t];
+        }
+        for (int k = 0; k < 2; ++k) {               // holes, then particles
+            const int k1 = k * ne1, k2 = k * ne2;
+            for (int i = 0; i < ne2; ++i) {
+                const int nit1 = itb[k2 + i];
+                bool found = false;
+                for (int j = k1; j < k1 + ne1; ++j) {
+                    if (ita[j] != nit1) continue;
+                    if (i != j - k1) {
+                        ita[j] = ita[i + k1];
+                        ita[i + k1] = nit1;
+                        ysig = !ysig;
+                    }
+                    found = true;
+                    break;
+                }
+                if (!found) {
+                    ++nbdif;
+                    if (nbdif > 2) return 0.0;
+                    nv1[nbdif - 1] = nit1;
+                    natu1[nbdif - 1] = k + 1;
+                    natu2[nbdif - 1] = i;
+                }
+            }
+        }
+    }
+    if (nbdif <= 0) return 0.0;
+    const int nar = ne1 - ne2;
+    for (int i = 0; i < nar; ++i) {
+        nv1[i] = ita[ne1 - 1 - i];
+        nv2[i] = ita[2 * ne1 - 1 - i];
+    }
+    for (int i = nar; i < nbdif; ++i) {
+        const int k = natu1[i], ni = natu2[i], nu = nv1[i];
+        nv2[i] = ita[ni + ne1 * (k - 1)];
+        if (k == 1) {
+            nv1[i] = nv2[i];
+            nv2[i] = nu;
+            ysig = !ysig;
+        }
+    }
+    const int n1 = D.iorb[nv1[0] - 1], n2 = D.iorb[nv2[0] - 1];
+    const int ns1 = D.ispin[nv1[0] - 1], ns2 = D.ispin[nv2[0] - 1];
+    double ha = 0.0;
+    if (nbdif == 2) {
+        const int n3 = D.iorb[nv1[1] - 1], n4 = D.iorb[nv2[1] - 1];
+        const int ns4 = D.ispin[nv2[1] - 1];
+        if (ns1 == ns2) ha = ai1(mo, n1, n2, n3, n4);
+        if (ns1 == ns4) ha = ha - ai1(mo, n1, n4, n3, n2);
+    } else {
+        ha = fock[tri1s(n1, n2)];
+        for (int i = 0; i < ne1; ++i) {
+            int np = D.iorb[ita[i] - 1];
+            int nsu = D.ispin[ita[i] - 1];
+            ha = ha - ai1(mo, n1, n2, np, np);
+            if (ns1 == nsu) ha = ha + ai1(mo, n1, np, n2, np);
+            np = D.iorb[ita[i + ne1] - 1];
+            if (np > norb) continue;
+            nsu = D.ispin[ita[i + ne1] - 1];
+            ha = ha + ai1(mo, n1, n2, np, np);
+            if (ns1 == nsu) ha = ha - ai1(mo, n1, np, n2, np);
+        }
+    }
+    return ysig ? -ha : ha;
+}
+
+__global__ void k_hntd(DetArrays D, int norb, const int* __restrict__ pairs, int64_t n_pairs,
+                       int n_det, const double* __restrict__ fock,
+                       const double* __restrict__ mo, double* __restrict__ out,
+                       unsigned long long* __restrict__ n_bad) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_pairs; q += stride) {
+        const int da = pairs[2 * q], db = pairs[2 * q + 1];
+        if (da < 0 || db < 0 || da >= n_det || db >= n_det) {
+            out[q] = __builtin_nan("");
+            atomicAdd(n_bad, 1ull);
+            continue;
+        }
+        out[q] = hntd_pair(D, da, db, norb, fock, mo);
+    }
+}
+
+
+int sc_validate(cipsi_ijkl_ctx* ctx, const DetArrays& D, int n_det, int64_t n_exc, int n_mo,
+                unsigned long long* n_bad) {
+    if (hipMemsetAsync(n_bad, 0, sizeof(unsigned long long), ctx->stream) != hipSuccess)
+        return CIPSI_ERR_HIP;
+    const int64_t work = n_det > 2 * n_mo ? n_det : 2 * n_mo;
+    const unsigned grid = (unsigned)((work + 255) / 256 < 65536 ? (work + 255) / 256 : 65536);
+    hipLaunchKernelGGL(k_sc_validate, dim3(grid), dim3(256), 0, ctx->stream, D, n_det, n_exc,
+                       2 * n_mo, n_mo, n_bad);
+    return hipGetLastError() == hipSuccess ? CIPSI_OK : CIPSI_ERR_HIP;
+}
+
+int sc_finish(cipsi_ijkl_ctx* ctx, unsigned long long* n_bad, const char* what) {
+    unsigned long long bad = 0;
+    if (hipMemcpyAsync(&bad, n_bad, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return CIPSI_ERR_HIP;
+    if (bad) {
+        ctx->last_error = std::string(what) + ": " + std::to_string(bad) +
+                          " determinants, spin-orbitals or pairs out of range";
+        return CIPSI_ERR_ARG;
+    }
+    return CIPSI_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -257,6 +469,62 @@ int cipsi_ijkl_fock(cipsi_ijkl_ctx* ctx, int n_mo, int noc, const double* mo_pac
         hipMemcpyAsync(kint, d_k, s_p, hipMemcpyDeviceToHost, st) != hipSuccess)
         return CIPSI_ERR_HIP;
     return hipStreamSynchronize(st) == hipSuccess ? CIPSI_OK : CIPSI_ERR_HIP;
+}
+
+
+int cipsi_ijkl_hdig_dev(cipsi_ijkl_ctx* ctx, int n_mo, int n_det, int64_t n_exc,
+                        const int* d_ne, const int* d_nd, const int* d_trou, const int* d_part,
+                        const int* d_iorb, const int* d_ispin, const double* d_fock,
+                        const double* d_jint, const double* d_kint, double* d_out) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if (n_mo < 1 || n_det < 0 || n_exc < 0 || !d_iorb || !d_ispin || !d_fock || !d_jint ||
+        !d_kint || (n_det > 0 && (!d_ne || !d_nd || !d_out)) ||
+        (n_exc > 0 && (!d_trou || !d_part)))
+        return CIPSI_ERR_ARG;
+    if (n_mo > kMaxMo) return CIPSI_ERR_UNSUPPORTED;
+    unsigned long long* n_bad;
+    double* e;
+    if ((rc = cons_scalars(ctx, &n_bad, &e))) return rc;
+    const DetArrays D{d_ne, d_nd, d_trou, d_part, d_iorb, d_ispin};
+    if ((rc = sc_validate(ctx, D, n_det, n_exc, n_mo, n_bad))) return rc;
+    if ((rc = sc_finish(ctx, n_bad, "hdig"))) return rc;
+    if (n_det > 0) {
+        StageTimer tm(ctx, "hdig");
+        hipLaunchKernelGGL(k_hdig, dim3((n_det + 255) / 256), dim3(256), 0, ctx->stream, D, n_det,
+                           d_fock, d_jint, d_kint, d_out);
+        if (hipGetLastError() != hipSuccess) return CIPSI_ERR_HIP;
+    }
+    return hipStreamSynchronize(ctx->stream) == hipSuccess ? CIPSI_OK : CIPSI_ERR_HIP;
+}
+
+int cipsi_ijkl_hntd_dev(cipsi_ijkl_ctx* ctx, int n_mo, int n_det, int64_t n_exc,
+                        const int* d_ne, const int* d_nd, const int* d_trou, const int* d_part,
+                        const int* d_iorb, const int* d_ispin, const int* d_pairs,
+                        int64_t n_pairs, const double* d_mo_packed, const double* d_fock,
+                        double* d_out) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if (n_mo < 1 || n_det < 0 || n_exc < 0 || n_pairs < 0 || !d_iorb || !d_ispin || !d_fock ||
+        !d_mo_packed || (n_det > 0 && (!d_ne || !d_nd)) ||
+        (n_exc > 0 && (!d_trou || !d_part)) || (n_pairs > 0 && (!d_pairs || !d_out)))
+        return CIPSI_ERR_ARG;
+    if (n_mo > kMaxMo) return CIPSI_ERR_UNSUPPORTED;
+    unsigned long long* n_bad;
+    double* e;
+    if ((rc = cons_scalars(ctx, &n_bad, &e))) return rc;
+    const DetArrays D{d_ne, d_nd, d_trou, d_part, d_iorb, d_ispin};
+    if ((rc = sc_validate(ctx, D, n_det, n_exc, n_mo, n_bad))) return rc;
+    if ((rc = sc_finish(ctx, n_bad, "hntd"))) return rc;
+    if (n_pairs > 0) {
+        StageTimer tm(ctx, "hntd");
+        const int64_t blocks = (n_pairs + 255) / 256;
+        hipLaunchKernelGGL(k_hntd, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0,
+                           ctx->stream, D, n_mo, d_pairs, n_pairs, n_det, d_fock, d_mo_packed,
+                           d_out, n_bad);
+        if (hipGetLastError() != hipSuccess) return CIPSI_ERR_HIP;
+    }
+    return sc_finish(ctx, n_bad, "hntd");
 }
 
 }  // extern "C"

@@ -49,6 +49,7 @@ EXPORTS = [
     "cipsi_ijkl_batch_autotune", "cipsi_ijkl_batch_run", "cipsi_ijkl_batch_num_classes",
     "cipsi_ijkl_batch_class_info", "cipsi_ijkl_batch_set_class_plan",
     "cipsi_ijkl_gather_dev", "cipsi_ijkl_fock", "cipsi_ijkl_fock_dev",
+    "cipsi_ijkl_hdig_dev", "cipsi_ijkl_hntd_dev",
 ]
 
 
@@ -123,6 +124,8 @@ def load_library(path: str = LIB_PATH):
     L.cipsi_ijkl_gather_dev.argtypes = [_vp, _c_int, _vp, _vp, _c_i64, _vp]
     L.cipsi_ijkl_fock.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]
     L.cipsi_ijkl_fock_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.cipsi_ijkl_hdig_dev.argtypes = [_vp, _c_int, _c_int, _c_i64] + [_vp] * 10
+    L.cipsi_ijkl_hntd_dev.argtypes = [_vp, _c_int, _c_int, _c_i64] + [_vp] * 7 + [_c_i64, _vp, _vp, _vp]
     _lib = L
     return L
 
@@ -410,6 +413,49 @@ class Context:
         self._check(self.lib.cipsi_ijkl_fock(self.handle, n_mo, noc, _ptr(arrs[0]), _ptr(arrs[1]),
                                              ctypes.byref(e), _ptr(f), _ptr(j), _ptr(k)), "fock")
         return e.value, f, j, k
+
+    def hdig(self, dets, n_mo, fock, jint, kint):
+        """E(D) - E(reference) for every determinant (cip/hdig.f), float64 CUDA tensor."""
+        import torch
+        out = torch.empty(dets.ne.numel(), dtype=torch.float64, device=dets.ne.device)
+        args = [x.contiguous() for x in (fock, jint, kint)]
+        self._use_torch_stream()
+        self._check(self.lib.cipsi_ijkl_hdig_dev(
+            self.handle, n_mo, dets.ne.numel(), dets.trou.numel(), *dets._args(),
+            *[_ptr(x) for x in args], _ptr(out)), "hdig")
+        return out
+
+    def hntd(self, dets, n_mo, pairs, mo_packed, fock):
+        """<D_a|H|D_b> for each row (a, b) of pairs (int32 CUDA tensor, 0-based), as
+        cip/hntd.f computes it; float64 CUDA tensor."""
+        import torch
+        pairs = pairs.contiguous().view(-1)
+        if pairs.dtype != torch.int32 or pairs.numel() % 2:
+            raise ValueError("hntd: pairs must be an int32 tensor of (a, b) rows")
+        n_pairs = pairs.numel() // 2
+        out = torch.empty(n_pairs, dtype=torch.float64, device=pairs.device)
+        mo_packed, fock = mo_packed.contiguous(), fock.contiguous()
+        self._use_torch_stream()
+        self._check(self.lib.cipsi_ijkl_hntd_dev(
+            self.handle, n_mo, dets.ne.numel(), dets.trou.numel(), *dets._args(), _ptr(pairs),
+            n_pairs, _ptr(mo_packed), _ptr(fock), _ptr(out)), "hntd")
+        return out
+
+
+class Determinants:
+    """cip's determinant arrays on the device (common /spo/, numbers 1-based): ne[d]
+    excitations of the closed-shell reference, holes trou[nd[d]:nd[d]+ne[d]] and
+    particles part[...] as spin-orbital numbers; spin-orbital s has orbital iorb[s-1] and
+    spin ispin[s-1].  Accepts array-likes; keeps int32 CUDA tensors."""
+
+    def __init__(self, ne, nd, trou, part, iorb, ispin, device="cuda"):
+        import torch
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
+        self.ne, self.nd, self.trou, self.part = t(ne), t(nd), t(trou), t(part)
+        self.iorb, self.ispin = t(iorb), t(ispin)
+
+    def _args(self):
+        return [_ptr(x) for x in (self.ne, self.nd, self.trou, self.part, self.iorb, self.ispin)]
 
 
 class BatchPlan:

@@ -221,6 +221,32 @@ int cipsi_ijkl_fock_dev(cipsi_ijkl_ctx *ctx, int n_mo, int noc, const double *d_
                         const double *d_h_mo_packed, double *d_energy, double *d_fock,
                         double *d_j, double *d_k);
 
+/* Slater-Condon elements of cip between determinants held, as cip holds them (common
+ * /spo/), as excitations of the closed-shell reference; all numbers 1-based:
+ *   determinant d (0-based): ne[d] <= 10 excitations, holes trou[nd[d] + t] and particles
+ *   part[nd[d] + t] (t < ne[d]) as spin-orbital numbers 1..2 n_mo; spin-orbital s has
+ *   orbital iorb[s-1] (1..n_mo) and spin ispin[s-1] (1 or 2).  n_exc = length of trou
+ *   and part.  fock, jint, kint: the packed F, J, K of cipsi_ijkl_fock (what cip reads
+ *   from fock's unit 40, cip/ijkf.f:26-36).
+ * hdig: out[d] = E(D_d) - E(reference) (cip/hdig.f:40-80, fdiag = diagonal of F).
+ * hntd: out[q] = <D_a|H|D_b> for pairs[2q] = a, pairs[2q+1] = b (0-based), zero when the
+ *   determinants are identical or differ by more than two spin-orbitals
+ *   (cip/hntd.f:36-168), with ai() on the packed MO stream.  cip keeps that stream in
+ *   REAL*4 (cip/ai.f); this reads the FP64 stream.
+ * Device pointers; the arrays are validated first (CIPSI_ERR_ARG on a number out of
+ * range or more than 10 excitations).  Synchronous at return.  No reference entry point
+ * is replaced: cip calls hdig / hntd one determinant or pair at a time (ic001.f:105,
+ * pert1.f:116, diexcit.f:293). */
+int cipsi_ijkl_hdig_dev(cipsi_ijkl_ctx *ctx, int n_mo, int n_det, int64_t n_exc,
+                        const int *d_ne, const int *d_nd, const int *d_trou, const int *d_part,
+                        const int *d_iorb, const int *d_ispin, const double *d_fock,
+                        const double *d_jint, const double *d_kint, double *d_out);
+int cipsi_ijkl_hntd_dev(cipsi_ijkl_ctx *ctx, int n_mo, int n_det, int64_t n_exc,
+                        const int *d_ne, const int *d_nd, const int *d_trou, const int *d_part,
+                        const int *d_iorb, const int *d_ispin, const int *d_pairs,
+                        int64_t n_pairs, const double *d_mo_packed, const double *d_fock,
+                        double *d_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,11 @@ def lib():
                                      _dp, _dp, _dp, _dp]
         L.or_fock.restype = ctypes.c_double
         L.or_fock.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        _ip32 = ctypes.POINTER(ctypes.c_int)
+        L.or_hdig.restype = ctypes.c_double
+        L.or_hdig.argtypes = [ctypes.c_int] + [_ip32] * 6 + [_dp, _dp, _dp]
+        L.or_hntd.restype = ctypes.c_double
+        L.or_hntd.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_ip32] * 6 + [_dp, _dp]
         _LIB = L
     return _LIB
 
@@ -137,6 +142,39 @@ def fock(mo_packed, hmo_packed, n_mo, noc):
     f, j, k = np.empty(pm), np.empty(pm), np.empty(pm)
     e = lib().or_fock(n_mo, noc, _d(_f64(mo_packed)), _d(_f64(hmo_packed)), _d(f), _d(j), _d(k))
     return e, f, j, k
+
+
+class Determinants:
+    """cip's determinant arrays (CIPSI_sources/cip, common /spo/), 1-based numbers:
+    ne[d] excitations of the closed-shell reference, holes trou[nd[d]:nd[d]+ne[d]] and
+    particles part[...] as spin-orbital numbers; spin-orbital s has orbital iorb[s-1]
+    and spin ispin[s-1]."""
+
+    def __init__(self, ne, nd, trou, part, iorb, ispin):
+        as32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        self.ne, self.nd, self.trou, self.part = as32(ne), as32(nd), as32(trou), as32(part)
+        self.iorb, self.ispin = as32(iorb), as32(ispin)
+
+    def _args(self):
+        ip = ctypes.POINTER(ctypes.c_int)
+        return [a.ctypes.data_as(ip) for a in (self.ne, self.nd, self.trou, self.part,
+                                                self.iorb, self.ispin)]
+
+
+def hdig(dets, fock, jint, kint):
+    """E(D) - E(reference) of every determinant (cip/hdig.f:40-80)."""
+    fock, jint, kint = _f64(fock), _f64(jint), _f64(kint)
+    a = dets._args()
+    return np.array([lib().or_hdig(d, *a, _d(fock), _d(jint), _d(kint))
+                     for d in range(dets.ne.size)])
+
+
+def hntd(dets, pairs, norb, fock, mo_packed):
+    """<D_a|H|D_b> for each pair (a, b) of 0-based determinant indices (cip/hntd.f:36-168)."""
+    fock, mo = _f64(fock), _f64(mo_packed)
+    a = dets._args()
+    return np.array([lib().or_hntd(int(x), int(y), norb, *a, _d(fock), _d(mo))
+                     for x, y in np.asarray(pairs).reshape(-1, 2)])
 
 
 def energy_grad(ao_packed, h_packed, c, gamma, Gamma_packed):

@@ -15,6 +15,8 @@
  *   or_tijkld_c1   CIPSI_sources/ijkl/tijkld.f:28-88   (second half, Q3 + Q4)
  *   or_tmono       CIPSI_sources/ijkl/tmono.f:39-54    (one-electron transform)
  *   or_fock        CIPSI_sources/fock/fock.f:80-127    (closed-shell energy, Fock, J, K)
+ *   or_hdig        CIPSI_sources/cip/hdig.f:40-80       (determinant energy)
+ *   or_hntd        CIPSI_sources/cip/hntd.f:36-168      (off-diagonal Slater-Condon element)
  *
  * Every accumulation starts from 0.0 and adds in the same index order as the
  * Fortran loop; build with -ffp-contract=off so that no multiply-add is fused.
@@ -452,4 +454,150 @@ double or_fock(int n_mo, int noc, const double *mo, const double *hmo, double *f
         }
     free(f);
     return val;
+}
+
+/*
+ * Slater-Condon elements of cip between determinants held as excitations of the
+ * closed-shell reference (cip's /spo/ arrays, all numbers 1-based as there):
+ *   determinant d has ne[d] excitations: holes trou[nd[d] + t] and particles
+ *   part[nd[d] + t], t < ne[d], as spin-orbital numbers; spin-orbital s has orbital
+ *   iorb[s - 1] and spin ispin[s - 1].
+ *   fock, jint, kint: packed F, J = (ii|jj), K = (ij|ij) (or_fock, read by cip in
+ *   ijkf.f:26-36); mo: the packed MO stream, fetched as ai() fetches it.
+ * tri1(a, b) = a(a-1)/2 + b - 1 is the 0-based packed index of the 1-based pair (a, b)
+ * as cip's num(a) + b, without reordering (hdig.f uses it as written).
+ */
+static inline int64_t tri1(int64_t a, int64_t b) { return a * (a - 1) / 2 + b - 1; }
+static inline int64_t tri1s(int64_t a, int64_t b) { return a >= b ? tri1(a, b) : tri1(b, a); }
+
+/* hdig.f:40-80: E(D) - E(reference); fdiag(s) = F(iorb(s), iorb(s)) (ijkf.f:33-36). */
+double or_hdig(int d, const int *ne, const int *nd, const int *trou, const int *part,
+               const int *iorb, const int *ispin, const double *fock, const double *jint,
+               const double *kint)
+{
+    const int nei = ne[d];
+    if (nei == 0) return 0.0;
+    const int l1 = nd[d];
+    double ab = 0.0;
+    for (int l = l1; l < l1 + nei; ++l) {
+        const int s1p = part[l], s1t = trou[l];
+        const int n1p = iorb[s1p - 1], n1t = iorb[s1t - 1];
+        ab = ab + fock[tri1(n1p, n1p)] - fock[tri1(n1t, n1t)];
+        const int ns1t = ispin[s1t - 1], ns1p = ispin[s1p - 1];
+        for (int j = l1; j <= l; ++j) {
+            const int s2t = trou[j], s2p = part[j];
+            const int ns2t = ispin[s2t - 1], ns2p = ispin[s2p - 1];
+            const int n2t = iorb[s2t - 1], n2p = iorb[s2p - 1];
+            const int64_t n12t = tri1s(n1t, n2t), n12p = tri1s(n1p, n2p);
+            const int64_t n1tp = tri1(n2p, n1t), n2tp = tri1(n1p, n2t);
+            double ca = jint[n12t] + jint[n12p] - jint[n1tp] - jint[n2tp];
+            if (ns1t == ns2t) ca = ca - kint[n12t];
+            if (ns1p == ns2p) ca = ca - kint[n12p];
+            if (ns1t == ns2p) ca = ca + kint[n1tp];
+            if (ns1p == ns2t) ca = ca + kint[n2tp];
+            if (j == l) ca = 0.5 * ca;
+            ab = ab + ca;
+        }
+    }
+    return ab;
+}
+
+/* ai() of cip for C1 on the packed MO stream, 1-based orbitals (cip/ai.f:1-42). */
+static inline double ai1(const double *mo, int a, int b, int c, int e)
+{
+    return or_ai(mo, a - 1, b - 1, c - 1, e - 1);
+}
+
+/* hntd.f:36-168: <D_a|H|D_b> when the determinants differ by one or two spin-orbitals,
+ * 0 when they are identical or differ by more.  Steps: the determinant with more
+ * excitations is "1"; its excitation list is reordered to line up with the other's
+ * (each swap flips the sign); unmatched holes / particles give the differing
+ * spin-orbitals nv1 -> nv2; one difference: F(n1, n2) corrected for the holes and
+ * particles of determinant 1; two: (n1 n2|n3 n4) - (n1 n4|n3 n2) by spin. */
+double or_hntd(int da, int db, int norb, const int *ne, const int *nd, const int *trou,
+               const int *part, const int *iorb, const int *ispin, const double *fock,
+               const double *mo)
+{
+    int ita[20], itb[20], nv1[2] = {0, 0}, nv2[2] = {0, 0}, natu1[2] = {0, 0}, natu2[2] = {0, 0};
+    const int kr = (ne[da] - ne[db] <= 0) ? db : da;
+    const int kl = (kr == db) ? da : db;
+    const int ne1 = ne[kr];
+    if (ne1 == 0) return 0.0;
+    const int ne2 = ne[kl];
+    int nbdif = ne1 - ne2;
+    if (nbdif > 2 || ne1 > 10) return 0.0;
+    for (int t = 0; t < ne1; ++t) {
+        ita[t] = trou[nd[kr] + t];
+        ita[t + ne1] = part[nd[kr] + t];
+    }
+    int ysig = 0;
+    if (ne2 > 0) {
+        for (int t = 0; t < ne2; ++t) {
+            itb[t] = trou[nd[kl] + t];
+            itb[t + ne2] = part[nd[kl] + t];
+        }
+        for (int k = 0; k < 2; ++k) {           /* holes, then particles */
+            const int k1 = k * ne1, k2 = k * ne2;
+            for (int i = 0; i < ne2; ++i) {
+                const int nit1 = itb[k2 + i];
+                int found = 0;
+                for (int j = k1; j < k1 + ne1; ++j) {
+                    if (ita[j] != nit1) continue;
+                    if (i != j - k1) {
+                        ita[j] = ita[i + k1];
+                        ita[i + k1] = nit1;
+                        ysig = !ysig;
+                    }
+                    found = 1;
+                    break;
+                }
+                if (!found) {
+                    nbdif = nbdif + 1;
+                    if (nbdif > 2) return 0.0;
+                    nv1[nbdif - 1] = nit1;
+                    natu1[nbdif - 1] = k + 1;
+                    natu2[nbdif - 1] = i;
+                }
+            }
+        }
+    }
+    if (nbdif <= 0) return 0.0;
+    const int nar = ne1 - ne2;                  /* extra excitations of determinant 1 */
+    for (int i = 0; i < nar; ++i) {
+        const int net = ne1 - 1 - i;
+        nv1[i] = ita[net];
+        nv2[i] = ita[net + ne1];
+    }
+    for (int i = nar; i < nbdif; ++i) {
+        const int k = natu1[i], ni = natu2[i], nu = nv1[i];
+        nv2[i] = ita[ni + ne1 * (k - 1)];
+        if (k == 1) {
+            nv1[i] = nv2[i];
+            nv2[i] = nu;
+            ysig = !ysig;
+        }
+    }
+    const int n1 = iorb[nv1[0] - 1], n2 = iorb[nv2[0] - 1];
+    const int ns1 = ispin[nv1[0] - 1], ns2 = ispin[nv2[0] - 1];
+    double ha = 0.0;
+    if (nbdif == 2) {
+        const int n3 = iorb[nv1[1] - 1], n4 = iorb[nv2[1] - 1];
+        const int ns4 = ispin[nv2[1] - 1];
+        if (ns1 == ns2) ha = ai1(mo, n1, n2, n3, n4);
+        if (ns1 == ns4) ha = ha - ai1(mo, n1, n4, n3, n2);
+    } else {
+        ha = fock[tri1s(n1, n2)];
+        for (int i = 0; i < ne1; ++i) {
+            int np = iorb[ita[i] - 1];
+            int nsu = ispin[ita[i] - 1];
+            ha = ha - ai1(mo, n1, n2, np, np);
+            if (ns1 == nsu) ha = ha + ai1(mo, n1, np, n2, np);
+            np = iorb[ita[i + ne1] - 1];
+            if (np > norb) continue;
+            nsu = ispin[ita[i + ne1] - 1];
+            ha = ha + ai1(mo, n1, n2, np, np);
+            if (ns1 == nsu) ha = ha - ai1(mo, n1, np, n2, np);
+        }
+    }
+    return ysig ? -ha : ha;
 }

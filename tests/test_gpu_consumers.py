@@ -1,4 +1,5 @@
-"""Integral consumers on the device (cipsi_ijkl_gather_dev, cipsi_ijkl_fock[_dev]).
+"""Integral consumers on the device (cipsi_ijkl_gather_dev, cipsi_ijkl_fock[_dev],
+cipsi_ijkl_hdig_dev, cipsi_ijkl_hntd_dev).
 
 * fock: on the same packed MO stream, the library's energy, Fock, J and K equal the
   oracle's or_fock bit for bit (both sum in fock.f's order without FMA); or_fock is pinned
@@ -10,6 +11,9 @@
   rotation among the occupied MOs (fock.f:80-101 depends on the occupied space only).
 * gather: ai(i,j,k,l) for labels in any index order, against a numpy lookup of the packed
   stream with the larger pair first (fock/ai.f:12-35); labels outside 1..n_mo are refused.
+* Slater-Condon (cip/hdig.f, cip/hntd.f): bit for bit against the oracle, which
+  tests/test_oracle_sc.py pins to brute-force second quantization; at N_MO = 240 chained
+  after the GPU transform and Fock build; out-of-range determinants and pairs are refused.
 """
 import numpy as np
 import pytest
@@ -147,3 +151,97 @@ def test_fock_after_full_size_transform(ctx):
     fv = synth.unpack_sym(f.cpu().numpy(), n)[noc:, noc:]
     fv2 = synth.unpack_sym(f2.cpu().numpy(), n)[noc:, noc:]
     assert np.max(np.abs(fv2 - fv)) <= 1e-11 * np.max(np.abs(fv))
+
+
+def _dets_from_lists(lists, m):
+    ne, nd, trou, part = [], [], [], []
+    for hs, ps in lists:
+        ne.append(len(hs))
+        nd.append(len(trou))
+        trou += [x + 1 for x in hs]
+        part += [x + 1 for x in ps]
+    return ne, nd, trou, part, [o + 1 for o in range(m)] * 2, [1] * m + [2] * m
+
+
+@pytest.mark.parametrize("m,noc,seed", [(5, 2, 500), (6, 3, 510)])
+def test_slater_condon_bitwise_equal_to_oracle(ctx, m, noc, seed):
+    """hdig / hntd on the device equal the oracle's (pinned to second quantization by
+    tests/test_oracle_sc.py) bit for bit, for every ordered pair of the determinant set,
+    identical determinants and pairs beyond two differences included."""
+    from cipsi_amd import ijkl
+    from tests.test_oracle_sc import _case
+    print(f"[slater-condon] m={m} noc={noc}", flush=True)
+    c = _case(m, noc, seed, 12)
+    arrays = _dets_from_lists(c["lists"], m)
+    odets = oracle.Determinants(*arrays)
+    gdets = ijkl.Determinants(*arrays)
+    f, j, k, mo = (torch.as_tensor(x, device="cuda") for x in (c["f"], c["j"], c["k"], c["mo"]))
+    hd = ctx.hdig(gdets, m, f, j, k)
+    np.testing.assert_array_equal(hd.cpu().numpy(), oracle.hdig(odets, c["f"], c["j"], c["k"]))
+    n = len(c["lists"])
+    pairs = np.array([(a, b) for a in range(n) for b in range(n)], dtype=np.int32)
+    ho = ctx.hntd(gdets, m, torch.as_tensor(pairs, device="cuda"), mo, f)
+    np.testing.assert_array_equal(ho.cpu().numpy(), oracle.hntd(odets, pairs, m, c["f"], c["mo"]))
+
+
+def test_slater_condon_rejects_bad_input(ctx):
+    from cipsi_amd import ijkl
+    m = 5
+    f = torch.zeros(synth.n_pairs(m), dtype=torch.float64, device="cuda")
+    mo = torch.zeros(synth.n_pairs(synth.n_pairs(m)), dtype=torch.float64, device="cuda")
+    good = _dets_from_lists([([], []), ([0], [3])], m)
+    for bad in ([1, 11], None):
+        arrays = list(good)
+        if bad is None:
+            arrays[3] = [2 * m + 1]                 # particle spin-orbital out of range
+        else:
+            arrays[0] = [0, 11]                     # more than 10 excitations
+        with pytest.raises(ijkl.IjklError):
+            ctx.hdig(ijkl.Determinants(*arrays), m, f, f, f)
+    dets = ijkl.Determinants(*good)
+    with pytest.raises(ijkl.IjklError):
+        ctx.hntd(dets, m, torch.tensor([[0, 2]], dtype=torch.int32, device="cuda"), mo, f)
+    ho = ctx.hntd(dets, m, torch.tensor([[0, 1], [1, 1]], dtype=torch.int32, device="cuda"), mo, f)
+    assert ho.cpu().tolist()[1] == 0.0             # identical determinants
+
+
+def test_slater_condon_after_full_size_fock(ctx):
+    """N_AO = N_MO = 240, 60 doubly occupied MOs: the GPU transform, Fock build and
+    Slater-Condon elements in a chain; 4000 determinants with up to 4 excitations and
+    their single / double neighbours; sampled pairs against the oracle on the same F and
+    stream, bit for bit."""
+    from cipsi_amd import ijkl
+    n, noc = 240, 60
+    print("[slater-condon-full]", flush=True)
+    B = synth.eri_factors(n, 3 * n, 20.0, 2030)
+    c = synth.orthonormal(n, 2031)
+    mo = ctx.transform(synth.eri_packed_torch(B, "cuda"), torch.as_tensor(c, device="cuda"))
+    hmo = ctx.tmono(synth.symmetric_packed(n, 2032), c)
+    _, f, j, k = ctx.fock(mo, torch.as_tensor(hmo, device="cuda"), n, noc)
+    rng = np.random.default_rng(2033)
+    lists = []
+    for d in range(2000):
+        ne = int(rng.integers(0, 5))
+        hs = list(rng.choice(noc, ne, replace=False))
+        sp = rng.integers(0, 2, ne)
+        ps = list(rng.choice(np.arange(noc, n), ne, replace=False))
+        lists.append(([h + n * s for h, s in zip(hs, sp)], [p + n * s for p, s in zip(ps, sp)]))
+        if ne:
+            ps2 = list(lists[-1][1])
+            ps2[-1] = int(rng.integers(noc, n)) + n * int(sp[-1])
+            if ps2[-1] not in lists[-1][1]:
+                lists.append((lists[-1][0], ps2))
+    arrays = _dets_from_lists(lists, n)
+    gdets = ijkl.Determinants(*arrays)
+    nd = len(lists)
+    pairs = np.stack([rng.integers(0, nd, 200000), rng.integers(0, nd, 200000)], 1)
+    pairs[:nd - 1] = np.stack([np.arange(nd - 1), np.arange(1, nd)], 1)   # neighbours
+    pairs = pairs.astype(np.int32)
+    ho = ctx.hntd(gdets, n, torch.as_tensor(pairs, device="cuda"), mo, f).cpu().numpy()
+    hd = ctx.hdig(gdets, n, f, j, k).cpu().numpy()
+    odets = oracle.Determinants(*arrays)
+    f_h, j_h, k_h, mo_h = f.cpu().numpy(), j.cpu().numpy(), k.cpu().numpy(), mo.cpu().numpy()
+    np.testing.assert_array_equal(hd, oracle.hdig(odets, f_h, j_h, k_h))
+    sample = np.concatenate([np.arange(nd - 1), rng.integers(nd, pairs.shape[0], 3000)])
+    np.testing.assert_array_equal(ho[sample], oracle.hntd(odets, pairs[sample], n, f_h, mo_h))
+    assert np.count_nonzero(ho[:nd - 1]) > nd // 4          # single / double neighbours interact
