@@ -233,9 +233,10 @@ def run_labels(args, torch):
 def run_consumers(args, torch):
     """Integral consumers on the device after one N_MO = 240 transform: the batched ai()
     gather over 2^27 uniformly random labels (HBM random access: 4 B label + 8 B
-    integral read + 8 B written per lookup, algorithmic) and the closed-shell Fock build
-    of fock.f:80-127 with N_MO / 4 doubly occupied MOs, beside the oracle's serial Fock
-    build on one host core."""
+    integral read + 8 B written per lookup, algorithmic), the closed-shell Fock build
+    of fock.f:80-127 with N_MO / 4 doubly occupied MOs (beside the oracle's serial Fock
+    build on one host core), and cip's Slater-Condon elements (hdig, hntd) over 20,000
+    determinants and 2^22 pairs."""
     import oracle
     from cipsi_amd import formats, ijkl, synth
     n = m = args.n_mo
@@ -270,8 +271,46 @@ def run_consumers(args, torch):
         e, f, j, k = ctx.fock(mo, d_hmo, m, noc)
     torch.cuda.synchronize()
     dt_f = (time.perf_counter() - t0) / args.steps
+    # Slater-Condon elements of cip (hdig, hntd) on the same F, J, K and stream:
+    # determinants with up to 4 excitations of the reference and a single-excitation
+    # neighbour each; half the pairs are neighbours, half are random
+    rng = np.random.default_rng(2030)
+    lists = []
+    for d in range(20000):
+        ne = int(rng.integers(0, 5))
+        hs = rng.choice(noc, ne, replace=False)
+        sp = rng.integers(0, 2, ne)
+        ps = rng.choice(np.arange(noc, m), ne, replace=False)
+        hl, pl = [int(a + m * b) for a, b in zip(hs, sp)], [int(a + m * b) for a, b in zip(ps, sp)]
+        lists.append((hl, pl))
+        if ne:
+            p2 = list(pl)
+            p2[-1] = int(rng.integers(noc, m)) + m * int(sp[-1])
+            if p2[-1] not in pl:
+                lists.append((hl, p2))
+    ne_, nd_, tr_, pa_ = [], [], [], []
+    for hl, pl in lists:
+        ne_.append(len(hl))
+        nd_.append(len(tr_))
+        tr_ += [x + 1 for x in hl]
+        pa_ += [x + 1 for x in pl]
+    dets = ijkl.Determinants(ne_, nd_, tr_, pa_, [o + 1 for o in range(m)] * 2, [1] * m + [2] * m)
+    n_sc = 1 << 22
+    nd = len(lists)
+    half = np.arange(n_sc // 2) % (nd - 1)
+    pairs = np.concatenate([np.stack([half, half + 1], 1),
+                            np.stack([rng.integers(0, nd, n_sc // 2),
+                                      rng.integers(0, nd, n_sc // 2)], 1)]).astype(np.int32)
+    d_pairs = torch.as_tensor(pairs, device="cuda")
+    sc_out = ctx.hntd(dets, m, d_pairs, mo, f)
+    ctx.hdig(dets, m, f, j, k)
+    for _ in range(args.steps):
+        sc_out = ctx.hntd(dets, m, d_pairs, mo, f)
+        ctx.hdig(dets, m, f, j, k)
     g_tot, g_cnt = ctx.read_timing("gather")
     f_tot, f_cnt = ctx.read_timing("fock")
+    n_tot, n_cnt = ctx.read_timing("hntd")
+    d_tot, d_cnt = ctx.read_timing("hdig")
     ctx.enable_timing(False)
     g_ms, f_ms = g_tot / g_cnt, f_tot / f_cnt
     gbs = 20.0 * n_q / (g_ms * 1e-3) / 1e9
@@ -291,6 +330,12 @@ def run_consumers(args, torch):
                                      else None)},
         "fock": {"n_mo": m, "noc": noc, "event_ms": f_ms, "host_ms": dt_f * 1e3,
                  "energy": float(e)},
+        "slater_condon": {
+            "determinants": nd, "pairs": n_sc,
+            "hntd_event_ms": n_tot / n_cnt, "hntd_pairs_per_s": n_sc / (n_tot / n_cnt * 1e-3),
+            "hntd_nonzero_frac": float((sc_out != 0).float().mean()),
+            "hdig_event_ms": d_tot / d_cnt,
+            "hdig_determinants_per_s": nd / (d_tot / d_cnt * 1e-3)},
     }
     if not args.no_cpu:
         mo_h = mo.cpu().numpy()

@@ -88,6 +88,24 @@ module cipsi_ijkl_mod
       real(c_double), intent(in) :: mo_packed(*), h_mo_packed(*)
       real(c_double), intent(out) :: energy, fock(*), jint(*), kint(*)
     end function
+
+    ! cip's hdig / hntd (cip/hdig.f, cip/hntd.f) for batches of determinants held in
+    ! cip's /spo/ arrays (ne, nd, trou, part, iorb, ispin; 1-based numbers), all device
+    ! buffers; out(d) = E(D_d) - E(reference), out(q) = <D_a|H|D_b> for pairs(2q-1:2q).
+    integer(c_int) function cipsi_ijkl_hdig_dev(ctx, n_mo, n_det, n_exc, ne, nd, trou, part, &
+        iorb, ispin, fock, jint, kint, out) bind(C, name="cipsi_ijkl_hdig_dev")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx, ne, nd, trou, part, iorb, ispin, fock, jint, kint, out
+      integer(c_int), value :: n_mo, n_det
+      integer(c_int64_t), value :: n_exc
+    end function
+    integer(c_int) function cipsi_ijkl_hntd_dev(ctx, n_mo, n_det, n_exc, ne, nd, trou, part, &
+        iorb, ispin, pairs, n_pairs, mo_packed, fock, out) bind(C, name="cipsi_ijkl_hntd_dev")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx, ne, nd, trou, part, iorb, ispin, pairs, mo_packed, fock, out
+      integer(c_int), value :: n_mo, n_det
+      integer(c_int64_t), value :: n_exc, n_pairs
+    end function
   end interface
 
 contains
