@@ -1,7 +1,7 @@
 // C ABI of the ijkl transform library (declarations: include/cipsi_ijkl.h).
 //
 // The pipeline for one C1 transform (all on one HIP stream, device-resident):
-//   pad_c        C -> zero-padded row-major copy used by the MFMA tiles
+//   pad_c        C -> zero-padded copy in 16-column blocks used by the MFMA tiles
 //   expand       packed AO block -> full symmetric pair matrix (rows contiguous)
 //   half 1       per AO pair: C^T X C lower triangle   (tpqkld.f)      -> H[P_A][P_M]
 //   transpose    H -> H^T (MO-pair rows)               (replaces unit 40, itijkl.f:249-278)

@@ -84,7 +84,6 @@ hipError_t launch_transpose(const double* src, int64_t rows, int64_t cols, doubl
 hipError_t launch_pad_c(const double* c, int ldc, int n, int m, int np, int mp, double* cr,
                         hipStream_t st);
 hipError_t launch_half_fp64(const HalfArgs& a, int variant, hipStream_t st);
-hipError_t launch_zero(double* p, int64_t n, hipStream_t st);
 
 // Row class (compile-time row-block count) used for a padded AO dimension np.
 int row_class(int np);
@@ -109,7 +108,7 @@ struct LayoutGeom {
     const double* c;       // caller's C, column-major, ld ldc
     double* vsq;           // full AO supermatrix, later H^T
     double* h;             // H
-    double* cr;            // padded row-major C
+    double* cr;            // zero-padded C in 16-column blocks (cr_index)
     int64_t pa, pm;
     int n, m, np, mp, ldc;
     int64_t expand_start;  // first expand tile of this transform

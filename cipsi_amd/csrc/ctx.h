@@ -23,7 +23,7 @@ struct cipsi_ijkl_ctx {
     // workspace (grow-only, device)
     DevBuf vsq;      // full symmetric AO pair matrix, later H^T
     DevBuf h;        // H[P_A][P_M]
-    DevBuf cr;       // padded row-major C
+    DevBuf cr;       // zero-padded C in 16-column blocks (cr_index)
     DevBuf packed;   // packed AO block built from labels
     DevBuf owner;    // last-writer election for duplicate labels
     DevBuf scal;     // small device scalars (first zero label, error flag)

@@ -16,9 +16,9 @@
 // O^T(c, a) = sum_r T(r, c) C(r, a) with T as the A operand, so a lane's results are
 // 16 consecutive a of one c -- 128-byte coalesced stores into the packed output.
 //
-// Z_b panels (KC columns s of the symmetric matrix, all rows) and the matching rows
-// of C are staged through LDS with register prefetch of the next panel; in Q2 the
-// 16-column blocks of C are staged the same way and shared by the waves.
+// Only the Z_b panels (KC columns s of the symmetric matrix, all rows) go through LDS,
+// prefetched into registers one panel ahead; the C operands of both products are read
+// from the zero-padded copy of C in global memory (L2-resident).
 #include "common.h"
 
 namespace cipsi {

@@ -3,18 +3,13 @@
 //  unpack_labels  : pshf label stream -> packed AO block            (transd.f, C1 route)
 //  expand_packed  : packed lower-triangular AO block -> full symmetric pair matrix
 //  transpose      : H[pq][kl] -> H^T[kl][pq] between the two halves
-//  pad_c          : C (column-major, ldc) -> zero-padded row-major copy used by the MFMA tiles
+//  pad_c          : C (column-major, ldc) -> zero-padded copy in 16-column blocks of kCRows
+//                   rows (cr_index) read by the MFMA tiles
 #include "common.h"
 
 namespace cipsi {
 
 // ----------------------------------------------------------------------------- fill
-__global__ void k_fill_f64(double* __restrict__ p, int64_t n, double v) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = v;
-}
-
 __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -27,11 +22,6 @@ static inline unsigned grid_for(int64_t n, int threads, unsigned cap = 65536u) {
     return (unsigned)(g > cap ? cap : g);
 }
 
-hipError_t launch_zero(double* p, int64_t n, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n, 256)), dim3(256), 0, st, p, n, 0.0);
-    return hipGetLastError();
-}
 
 // ----------------------------------------------------------------------------- labels
 // transd.f:85-97 (decode, first zero label ends the stream), 210-218 (doubling of
