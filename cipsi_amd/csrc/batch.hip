@@ -3,8 +3,8 @@
 // The reference transforms one geometry per ijkl process, the scan driver starting
 // one process per geometry (Autocip_13/autocip13.f90:525-567, ijkl.f:318).  Here the
 // whole scan is one hipGraph launch:
-//   pad_c (all)  ->  expand (all)  ->  half 1, one persistent launch per row class
-//   ->  transpose (all)  ->  half 2, one persistent launch per row class
+//   pad_c (all)  ->  expand (all)  ->  half 1, one launch per row class
+//   ->  transpose (all)  ->  half 2, one launch per row class
 // Every launch is grouped over the geometries: its kernels read the per-geometry
 // arguments from device tables, so the captured graph does not depend on the caller's
 // pointers and is reused by every run of the plan (a run uploads the tables, then
@@ -38,7 +38,6 @@ struct ClassPlan {
     std::vector<int> geoms;      // geometry indices, in scan order
     int first = 0;               // offset of this class in the per-class-ordered tables
     int v1 = 3, v2 = 3;          // tile variants of the two halves
-    int cap1 = 0, cap2 = 0;      // resident-capacity grids of the chosen variants
     int64_t items1 = 0, items2 = 0;
 };
 
@@ -205,11 +204,8 @@ GroupLaunch group_args(const cipsi_ijkl_batch* b, size_t k, int half) {
     gl.ngeom = (int)cp.geoms.size();
     gl.row_class = cp.row_class;
     gl.variant = half == 1 ? cp.v1 : cp.v2;
-    const int cap = half == 1 ? cp.cap1 : cp.cap2;
     const int64_t items = half == 1 ? cp.items1 : cp.items2;
-    const int64_t need = (items + 7) / 8 * 8;
-    gl.grid = (int)(need < cap ? need : cap);
-    gl.grid_out = nullptr;
+    gl.grid = (items + 7) / 8 * 8;
     return gl;
 }
 
@@ -218,11 +214,9 @@ int prepare_class(cipsi_ijkl_ctx* ctx, ClassPlan& cp) {
     gl.row_class = cp.row_class;
     gl.grid = 0;
     gl.variant = cp.v1;
-    gl.grid_out = &cp.cap1;
     hipError_t e = launch_half_fp64_group(gl, nullptr);
     if (e != hipSuccess) return fail(ctx, e, "batch prepare");
     gl.variant = cp.v2;
-    gl.grid_out = &cp.cap2;
     e = launch_half_fp64_group(gl, nullptr);
     if (e != hipSuccess) return fail(ctx, e, "batch prepare");
     return CIPSI_OK;

@@ -88,8 +88,8 @@ hipError_t launch_half_fp64(const HalfArgs& a, int variant, hipStream_t st);
 // Row class (compile-time row-block count) used for a padded AO dimension np.
 int row_class(int np);
 
-// A persistent grouped launch of one half over several transforms whose padded AO
-// dimension falls in one row class.  geoms[g] are the per-transform arguments (device
+// A grouped launch of one half over several transforms whose padded AO dimension
+// falls in one row class.  geoms[g] are the per-transform arguments (device
 // array); item_start (device, ngeom + 1 entries) numbers their work items.
 struct GroupLaunch {
     const HalfArgs* geoms;
@@ -97,8 +97,7 @@ struct GroupLaunch {
     int ngeom;
     int row_class;
     int variant;
-    int grid;        // > 0: launch with this grid; <= 0: prepare only
-    int* grid_out;   // prepare: resident-capacity grid (multiple of 8)
+    int64_t grid;    // > 0: launch with this grid (>= item total); <= 0: prepare only
 };
 hipError_t launch_half_fp64_group(const GroupLaunch& gl, hipStream_t st);
 

@@ -275,27 +275,28 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
     half_tile<NW, NRB, KC>(A, item);
 }
 
-// Persistent grouped launch over the work items of several transforms (a scan of
-// geometries).  Items are numbered geometry after geometry (item_start[g] is the
-// first item of geometry g, item_start[ngeom] the total); the item range is cut
-// into 8 contiguous pieces, one per XCD (work-group w runs on XCD w % 8), and the
-// work-groups of an XCD stride through its piece, so neighbouring tiles of one
-// matrix share an L2.  The grid is a multiple of 8 and sized to the resident
-// capacity of the device; every work-group leaves when its piece is done.
+// Grouped launch over the work items of several transforms (a scan of geometries).
+// Items are numbered geometry after geometry (item_start[g] is the first item of
+// geometry g, item_start[ngeom] the total); the grid holds one work-group per item, so
+// the hardware dispatcher balances the work as in a single transform's launch, and the
+// item order is cut into 8 contiguous pieces, one per XCD (work-group w runs on XCD
+// w % 8), so neighbouring tiles of one matrix share an L2.  The owning geometry is
+// found by binary search over the start table.
 template <int NW, int NRB, int KC, int MINB>
 __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64_group(const HalfArgs* __restrict__ geoms,
                                                                   const int64_t* __restrict__ item_start,
                                                                   int ngeom) {
     const int64_t total = item_start[ngeom];
-    const int x = blockIdx.x & 7;
-    const int64_t lo = total * x / 8, hi = total * (x + 1) / 8;
-    const int64_t nx = gridDim.x >> 3;
-    int g = 0;
-    for (int64_t it = lo + (blockIdx.x >> 3); it < hi; it += nx) {
-        while (item_start[g + 1] <= it) ++g;       // items only increase: walk forward
-        const HalfArgs A = geoms[g];
-        half_tile<NW, NRB, KC>(A, it - item_start[g]);
+    const int64_t per_xcd = gridDim.x >> 3;
+    const int64_t it = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (it >= total) return;
+    int lo = 0, hi = ngeom;                        // item_start[0] = 0 <= it < total
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (item_start[mid] <= it) lo = mid; else hi = mid;
     }
+    const HalfArgs A = geoms[lo];
+    half_tile<NW, NRB, KC>(A, it - item_start[lo]);
 }
 
 template <int NW, int NRB, int KC>
@@ -320,29 +321,16 @@ static hipError_t launch_v(const HalfArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-// Grouped launch.  grid <= 0 only prepares the kernel (LDS attribute) and returns in
-// *grid_out the resident-capacity grid (a multiple of 8); grid > 0 launches.
+// Grouped launch.  grid <= 0 only prepares the kernel (LDS attribute); grid > 0
+// launches (one work-group per item, rounded up to a multiple of 8).
 template <int NW, int NRB, int KC, int MINB>
 static hipError_t launch_group_v(const GroupLaunch& gl, hipStream_t st) {
     using K = HalfCfg<NW, NRB, KC>;
     const void* fn = (const void*)k_half_fp64_group<NW, NRB, KC, MINB>;
     const size_t shmem = half_shmem<NW, NRB, KC>();
-    if (gl.grid <= 0) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024);
-        if (e != hipSuccess) return e;
-        int per_cu = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, K::NT, shmem);
-        if (e != hipSuccess) return e;
-        if (per_cu < 1) per_cu = 1;
-        int dev = 0, ncu = 0;
-        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-        if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) !=
-            hipSuccess)
-            return e;
-        if (gl.grid_out) *gl.grid_out = (ncu * per_cu + 7) / 8 * 8;
-        return hipSuccess;
-    }
+    if (gl.grid <= 0)
+        return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (gl.grid > 0x7fffffff || (gl.grid & 7)) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_half_fp64_group<NW, NRB, KC, MINB>), dim3((unsigned)gl.grid), dim3(K::NT),
                        shmem, st, gl.geoms, gl.item_start, gl.ngeom);
     return hipGetLastError();

@@ -255,10 +255,12 @@ hipError_t launch_expand_packed(const double* ao_packed, int64_t pa, double* vsq
 }
 
 // ----------------------------------------------------------------------------- transpose
-__device__ __forceinline__ void transpose_tile(const double* __restrict__ src, int64_t rows,
-                                               int64_t cols, double* __restrict__ dst,
-                                               int64_t tiles_c, int64_t t) {
-    __shared__ double tile[XT][XT + 1];
+using XTile = double[XT][XT + 1];
+
+__device__ __forceinline__ void transpose_tile(XTile& tile, const double* __restrict__ src,
+                                               int64_t rows, int64_t cols,
+                                               double* __restrict__ dst, int64_t tiles_c,
+                                               int64_t t) {
     const int64_t ti = t / tiles_c, tj = t - ti * tiles_c;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int64_t r0 = ti * XT, c0 = tj * XT;
@@ -276,10 +278,10 @@ __device__ __forceinline__ void transpose_tile(const double* __restrict__ src, i
 // The same tile with 16-byte accesses: a thread moves two adjacent doubles of a row on
 // the way in and two adjacent doubles of a column on the way out.  Needs even rows and
 // cols (every pair then starts on a 16-byte boundary of 16-byte-aligned buffers).
-__device__ __forceinline__ void transpose_tile2(const double* __restrict__ src, int64_t rows,
-                                                int64_t cols, double* __restrict__ dst,
-                                                int64_t tiles_c, int64_t t) {
-    __shared__ double tile[XT][XT + 1];
+__device__ __forceinline__ void transpose_tile2(XTile& tile, const double* __restrict__ src,
+                                                int64_t rows, int64_t cols,
+                                                double* __restrict__ dst, int64_t tiles_c,
+                                                int64_t t) {
     const int64_t ti = t / tiles_c, tj = t - ti * tiles_c;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;      // 32 x 8
     const int64_t r0 = ti * XT, c0 = tj * XT;
@@ -301,13 +303,15 @@ __device__ __forceinline__ void transpose_tile2(const double* __restrict__ src, 
 __global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ src, int64_t rows,
                                                    int64_t cols, double* __restrict__ dst,
                                                    int64_t tiles_c) {
-    transpose_tile(src, rows, cols, dst, tiles_c, blockIdx.x);
+    __shared__ XTile tile;
+    transpose_tile(tile, src, rows, cols, dst, tiles_c, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_transpose2(const double* __restrict__ src, int64_t rows,
                                                     int64_t cols, double* __restrict__ dst,
                                                     int64_t tiles_c) {
-    transpose_tile2(src, rows, cols, dst, tiles_c, blockIdx.x);
+    __shared__ XTile tile;
+    transpose_tile2(tile, src, rows, cols, dst, tiles_c, blockIdx.x);
 }
 
 int64_t transpose_tiles(int64_t rows, int64_t cols) {
@@ -370,8 +374,16 @@ __global__ __launch_bounds__(256) void k_group_expand(const LayoutGeom* __restri
 __global__ __launch_bounds__(256) void k_group_transpose(const LayoutGeom* __restrict__ g, int ngeom) {
     const int64_t t = blockIdx.x;
     const int k = find_geom(g, ngeom, t, [](const LayoutGeom& x) { return x.trans_start; });
-    transpose_tile(g[k].h, g[k].pa, g[k].pm, g[k].vsq, (g[k].pm + XT - 1) / XT,
-                   t - g[k].trans_start);
+    const LayoutGeom& G = g[k];
+    const int64_t tc = (G.pm + XT - 1) / XT;
+    // the per-geometry buffers come from hipMalloc (aligned); even extents allow the
+    // 16-byte tile, and the choice is uniform over the work-group.  Both paths share one
+    // LDS tile so the kernel keeps the occupancy of a single-variant transpose.
+    __shared__ XTile tile;
+    if (G.pa % 2 == 0 && G.pm % 2 == 0)
+        transpose_tile2(tile, G.h, G.pa, G.pm, G.vsq, tc, t - G.trans_start);
+    else
+        transpose_tile(tile, G.h, G.pa, G.pm, G.vsq, tc, t - G.trans_start);
 }
 
 __global__ __launch_bounds__(256) void k_group_pad_c(const LayoutGeom* __restrict__ g, int ngeom,
