@@ -460,7 +460,7 @@ class Determinants:
 
 class BatchPlan:
     """A scan of geometries transformed together (cipsi_ijkl_batch_*): one hipGraph
-    launch of grouped persistent kernels per run.  Inputs are lists of float64 CUDA
+    launch of grouped kernels per run.  Inputs are lists of float64 CUDA
     tensors: packed AO blocks, C (N_AO x N_MO) and preallocated packed MO outputs."""
 
     def __init__(self, ctx: "Context", n_ao, n_mo):

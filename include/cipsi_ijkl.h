@@ -168,7 +168,7 @@ int cipsi_ijkl_reset_timing(cipsi_ijkl_ctx *ctx);
  * The reference transforms one geometry per ijkl process; its scan driver starts one
  * process per geometry (Autocip_13/autocip13.f90:525-567, each reaching ijkl.f:318).
  * A batch plan transforms all geometries of a scan, each with its own N_AO, N_MO, AO
- * block and C, in one hipGraph launch of grouped persistent kernels; the per-geometry
+ * block and C, in one hipGraph launch of grouped kernels; the per-geometry
  * results are those of cipsi_ijkl_transform_packed_dev.
  *   create      sizes only; allocates the device workspace of every geometry
  *               (cipsi_ijkl_batch_workspace_bytes) and per-geometry argument tables.

@@ -1,5 +1,5 @@
 """Batched scan (cipsi_ijkl_batch_*): every geometry of a scan, transformed by the grouped
-persistent kernels in one graph launch, equals the single-transform path on the same
+kernels in one graph launch, equals the single-transform path on the same
 inputs bit for bit (the tile variants and row classes only change which work-group
 computes an element, not the order of its sums), and the single path is pinned to the
 oracle and the closed form by tests/test_gpu_parity.py and tests/test_gpu_fullsize.py.
