@@ -223,7 +223,7 @@ def run_labels(args, torch):
         "config": {"workload": "transd C1 unpack, %d labels" % npk, "n_ao": n},
         "labels_per_s": p["labels_per_s"], "exact_vs_packed_block": p["exact_vs_packed_block"],
         "shuffled_stream": res.get("shuffled"),
-        "roofline": {"bound": "hbm", "kernel": "label unpack (find-first-zero, claim, store)",
+        "roofline": {"bound": "hbm", "kernel": "label unpack (owner clear, claim scan, resolve; all launches of one call)",
                      "achieved": round(p["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(p["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                      "event_ms": p["event_ms"]},

@@ -146,6 +146,21 @@ __device__ inline int64_t tri_row(int64_t x) {
 // One thread per four consecutive positions: the first is decomposed into
 // (ab, cd) = pair indices and (i, j), (k, l) = AO indices, the next three by stepping.
 // The doubling count is (i == j) + (k == l) + (ab == cd) (transd.f:210-218).
+// The owner quad of the thread's next four positions is loaded one iteration ahead, so
+// an owner load is in flight while the value gathers of the current quad wait.  With a
+// 16-byte-aligned output (VEC) a quad is stored as two 16-byte writes.
+__device__ __forceinline__ void load_owner4(const uint32_t* __restrict__ owner, int64_t p0,
+                                            int64_t nblk, uint32_t w[4]) {
+    if (p0 + 3 < nblk) {
+        const uint4 o = *(const uint4*)(owner + p0);             // owner is 16-byte aligned
+        w[0] = o.x; w[1] = o.y; w[2] = o.z; w[3] = o.w;
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w[e] = p0 + e < nblk ? owner[p0 + e] : 0u;
+    }
+}
+
+template <bool VEC>
 __global__ __launch_bounds__(kLabelThreads) void k_resolve(const double* __restrict__ xx,
                                                             int64_t nblk,
                                                             const uint32_t* __restrict__ owner,
@@ -156,33 +171,30 @@ __global__ __launch_bounds__(kLabelThreads) void k_resolve(const double* __restr
     if (blockIdx.x == 0 && threadIdx.x == 0 && sc[1] < n) *err_flag = 1;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t nq = (nblk + 3) >> 2;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    uint32_t w[4];
+    load_owner4(owner, 4 * q, nblk, w);
+    for (; q < nq; q += stride) {
         const int64_t p0 = 4 * q;
+        uint32_t wn[4] = {0u, 0u, 0u, 0u};
+        if (q + stride < nq) load_owner4(owner, 4 * (q + stride), nblk, wn);
+        double v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            v[e] = (w[e] != 0u && (uint64_t)(w[e] - 1) < n) ? xx[w[e] - 1] : 0.0;
         const int64_t ab64 = tri_row(p0);
         int ab = (int)ab64, cd = (int)(p0 - ab64 * (ab64 + 1) / 2);
         int i = (int)tri_row(ab), k = (int)tri_row(cd);
         int j = ab - i * (i + 1) / 2, l = cd - k * (k + 1) / 2;
-        uint32_t w[4];
-        if (p0 + 3 < nblk) {
-            const uint4 o = *(const uint4*)(owner + p0);         // owner is 16-byte aligned
-            w[0] = o.x; w[1] = o.y; w[2] = o.z; w[3] = o.w;
-        } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) w[e] = p0 + e < nblk ? owner[p0 + e] : 0u;
-        }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            if (p0 + e < nblk) {
-                double v = 0.0;
-                if (w[e] != 0u && (uint64_t)(w[e] - 1) < n) {
-                    v = xx[w[e] - 1];
-                    const int d = (i == j) + (k == l) + (ab == cd);
-                    if (d >= 1) v = v + v;
-                    if (d >= 2) v = v + v;
-                    if (d >= 3) v = v + v;
-                }
-                out[p0 + e] = v;
-            }
+            const int d = (i == j) + (k == l) + (ab == cd);
+            double x = v[e];
+            if (d >= 1) x = x + x;
+            if (d >= 2) x = x + x;
+            if (d >= 3) x = x + x;
+            v[e] = x;
             // step to the next position: cd, then (k, l); past the diagonal, the next ab
             if (++l > k) { ++k; l = 0; }
             if (++cd > ab) {
@@ -190,6 +202,16 @@ __global__ __launch_bounds__(kLabelThreads) void k_resolve(const double* __restr
                 if (++j > i) { ++i; j = 0; }
             }
         }
+        if (VEC && p0 + 3 < nblk) {
+            *(double2*)(out + p0) = make_double2(v[0], v[1]);
+            *(double2*)(out + p0 + 2) = make_double2(v[2], v[3]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (p0 + e < nblk) out[p0 + e] = v[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w[e] = wn[e];
     }
 }
 
@@ -207,8 +229,13 @@ hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t
         hipLaunchKernelGGL(k_fix_claim, dim3(grid_for(n_entries, 256)), dim3(256), 0, st, labels,
                            n_ao, owner, sc);
     }
-    hipLaunchKernelGGL(k_resolve, dim3(grid_for((nblk + 3) / 4, kLabelThreads)), dim3(kLabelThreads), 0, st,
-                       xx, nblk, owner, sc, err_flag, ao_packed);
+    const dim3 rg(grid_for((nblk + 3) / 4, kLabelThreads));
+    if ((uintptr_t)ao_packed % 16 == 0)
+        hipLaunchKernelGGL(k_resolve<true>, rg, dim3(kLabelThreads), 0, st, xx, nblk, owner, sc,
+                           err_flag, ao_packed);
+    else
+        hipLaunchKernelGGL(k_resolve<false>, rg, dim3(kLabelThreads), 0, st, xx, nblk, owner, sc,
+                           err_flag, ao_packed);
     return hipGetLastError();
 }
 
