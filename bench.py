@@ -96,6 +96,43 @@ def time_transform(ctx, ao, c_dev, prec, args, torch):
             "slices": (args.slices or 6) if prec == "split_fp16" else 0}
 
 
+def time_graph(ctx, ao, c_dev, args, torch):
+    """Capture one FP64 transform in a CUDA graph and time args.steps replays with HIP
+    events on the replay stream.  Returns (output of the last replay, ms per replay)."""
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):                 # workspace and tables exist before capture
+        ctx.transform(ao, c_dev)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = ctx.transform(ao, c_dev)
+    for _ in range(max(1, args.warmup)):
+        g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    del g
+    return out, ms
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def split_flops_executed(n, m, s):
     """FP16 MFMA flops the split kernels execute (tiles of 32 rows x 32-column wave
     blocks, K padded to 32; digit products s(s+1)/2)."""
@@ -527,6 +564,16 @@ def main():
                    "n_ao": n, "n_mo": m, "precision": main_prec, "mo_integrals_per_step": n_out},
         "roofline": roofline,
     }
+    if main_prec == "fp64":
+        gout, gms = time_graph(ctx, ao, c_dev, args, torch)
+        same = bool(torch.equal(gout, out))
+        result["graph_replay"] = {
+            "ms_per_step": gms, "value": n_out / (gms * 1e-3), "unit": "MO-integrals/s",
+            "timing": "HIP events around the replays of one captured transform",
+            "bit_identical_to_eager": same}
+        print(f"[bench] fp64 graph replay: {gms:.3f} ms/step (identical: {same})",
+              file=sys.stderr, flush=True)
+        del gout
     if "split_fp16" in runs and main_prec == "fp64":
         sp = runs["split_fp16"]
         err = float((sp["out"] - out).abs().max())
@@ -549,7 +596,7 @@ def main():
                                                   nthreads=args.cpu_threads)
         result["cpu_baseline"] = {
             "value": n_out / est, "unit": "MO-integrals/s", "cores": args.cpu_threads,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"oracle tpqkld on {s1} AO pairs + tijkld on {s2} MO pairs (stride-sampled) "
                       f"of the same N={n} workload, {spent:.1f} s measured, extrapolated to "
                       f"{est:.1f} s per transform; no disk I/O",

@@ -189,3 +189,35 @@ def test_label_stream_duplicates_and_stale_tail(ctx):
     assert e.value.status == -2
     got = ctx.transd(dev(xx), dev(ix), n)                        # still usable
     np.testing.assert_array_equal(got.cpu().numpy(), want)
+
+
+def test_transform_under_graph_capture(ctx):
+    """A caller can capture the device transform in a CUDA graph (the library launches
+    only kernels on the caller's stream once its workspace exists); each replay, with new
+    inputs copied in and eager work in between, equals the eager result bit for bit."""
+    n, m = 40, 24
+    print(f"[graph] n={n} m={m}", flush=True)
+    cases = [synth.make_case(n, m, 2 * n, 6.0, 700 + k) for k in range(3)]
+    V, c = dev(cases[0]["ao_packed"]), dev(cases[0]["c"])
+    refs = [ctx.transform(dev(k["ao_packed"]), dev(k["c"])) for k in cases]
+    torch.cuda.synchronize()
+    assert np.max(np.abs(refs[0].cpu().numpy() - oracle.ijkl(cases[0]["ao_packed"],
+                                                             cases[0]["c"]))) <= TOL
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):                 # warm-up: workspace and tables exist
+        ctx.transform(V, c)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = ctx.transform(V, c)
+    for rep in range(2):
+        for k, case in enumerate(cases):
+            V.copy_(dev(case["ao_packed"]))
+            c.copy_(dev(case["c"]))
+            scratch = torch.randn(1 << 20, device="cuda", dtype=torch.float64)
+            scratch.mul_(2.0)                  # eager work between replays
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(out, refs[k]), f"replay {rep} case {k}"
