@@ -14,6 +14,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cipsi_ijkl.h"
@@ -232,6 +233,7 @@ int cipsi_ijkl_release_workspace(cipsi_ijkl_ctx* ctx) {
     ctx_free(ctx->io_out);
     ctx_free(ctx->io_c);
     ctx_free(ctx->scal);
+    ctx_free(ctx->scan);
     ctx_free(ctx->tile_start);
     ctx_free(ctx->slices);
     ctx_free(ctx->adj_g);
@@ -242,10 +244,13 @@ int cipsi_ijkl_release_workspace(cipsi_ijkl_ctx* ctx) {
     return CIPSI_OK;
 }
 
+static void pin_free(cipsi_ijkl_ctx* ctx);
+
 int cipsi_ijkl_destroy(cipsi_ijkl_ctx* ctx) {
     if (!ctx) return CIPSI_ERR_ARG;
     cipsi_ijkl_reset_timing(ctx);
     cipsi_ijkl_release_workspace(ctx);
+    pin_free(ctx);
     ctx_rocblas_destroy(ctx);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
@@ -385,6 +390,107 @@ int cipsi_ijkl_half2_dev(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* 
 }
 
 // ------------------------------------------------------------------ host-pointer entry points
+// ------------------------------------------------------------------ host transfers
+// Large host <-> device copies of the host entry points go through four pinned 64 MiB
+// slots: host threads copy a chunk between the caller's (pageable) memory and a slot
+// while the DMA engine moves the previous slot, so the two overlap.  A slot is reused
+// only after the event of its last DMA has completed.
+static constexpr int kCopyThreads = 8;
+
+static void parallel_memcpy(void* dst, const void* src, size_t bytes) {
+    if (bytes < (size_t(4) << 20)) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = ((bytes / kCopyThreads) + 63) & ~size_t(63);
+    std::thread th[kCopyThreads];
+    int used = 0;
+    for (size_t off = 0; off < bytes; off += part) {
+        const size_t len = std::min(part, bytes - off);
+        th[used++] = std::thread([=] { std::memcpy((char*)dst + off, (const char*)src + off, len); });
+    }
+    for (int t = 0; t < used; ++t) th[t].join();
+}
+
+static int pin_ready(cipsi_ijkl_ctx* ctx) {
+    for (int s = 0; s < cipsi_ijkl_ctx::kPinSlots; ++s) {
+        if (!ctx->pin[s]) {
+            HIP_TRY(ctx, hipHostMalloc(&ctx->pin[s], cipsi_ijkl_ctx::kPinBytes, hipHostMallocDefault));
+            HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->pin_ev[s], hipEventDisableTiming));
+        }
+    }
+    return CIPSI_OK;
+}
+
+static int pin_wait(cipsi_ijkl_ctx* ctx, int s) {
+    if (ctx->pin_busy[s]) {
+        HIP_TRY(ctx, hipEventSynchronize(ctx->pin_ev[s]));
+        ctx->pin_busy[s] = false;
+    }
+    return CIPSI_OK;
+}
+
+static int copy_h2d(cipsi_ijkl_ctx* ctx, void* d, const void* h, size_t bytes) {
+    int rc = pin_ready(ctx);
+    if (rc) return rc;
+    const size_t chunk = cipsi_ijkl_ctx::kPinBytes;
+    int s = 0;
+    for (size_t off = 0; off < bytes; off += chunk, s = (s + 1) % cipsi_ijkl_ctx::kPinSlots) {
+        const size_t len = std::min(chunk, bytes - off);
+        if ((rc = pin_wait(ctx, s))) return rc;
+        parallel_memcpy(ctx->pin[s], (const char*)h + off, len);
+        HIP_TRY(ctx, hipMemcpyAsync((char*)d + off, ctx->pin[s], len, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->pin_ev[s], ctx->stream));
+        ctx->pin_busy[s] = true;
+    }
+    return CIPSI_OK;
+}
+
+// Ordered after the work already queued on the context stream; returns when the caller's
+// buffer holds the data.
+static int copy_d2h(cipsi_ijkl_ctx* ctx, void* h, const void* d, size_t bytes) {
+    int rc = pin_ready(ctx);
+    if (rc) return rc;
+    constexpr int S = cipsi_ijkl_ctx::kPinSlots;
+    const size_t chunk = cipsi_ijkl_ctx::kPinBytes;
+    const int64_t nch = (int64_t)((bytes + chunk - 1) / chunk);
+    auto issue = [&](int64_t k) -> int {
+        const int s = (int)(k % S);
+        const size_t off = (size_t)k * chunk, len = std::min(chunk, bytes - off);
+        int r = pin_wait(ctx, s);
+        if (r) return r;
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->pin[s], (const char*)d + off, len, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->pin_ev[s], ctx->stream));
+        ctx->pin_busy[s] = true;
+        return CIPSI_OK;
+    };
+    for (int64_t k = 0; k < nch && k < S; ++k)
+        if ((rc = issue(k))) return rc;
+    for (int64_t k = 0; k < nch; ++k) {
+        const int s = (int)(k % S);
+        const size_t off = (size_t)k * chunk, len = std::min(chunk, bytes - off);
+        if ((rc = pin_wait(ctx, s))) return rc;
+        parallel_memcpy((char*)h + off, ctx->pin[s], len);
+        if (k + S < nch && (rc = issue(k + S))) return rc;
+    }
+    return CIPSI_OK;
+}
+
+static void pin_free(cipsi_ijkl_ctx* ctx) {
+    for (int s = 0; s < cipsi_ijkl_ctx::kPinSlots; ++s) {
+        if (ctx->pin_ev[s]) {
+            (void)hipEventSynchronize(ctx->pin_ev[s]);
+            (void)hipEventDestroy(ctx->pin_ev[s]);
+            ctx->pin_ev[s] = nullptr;
+        }
+        if (ctx->pin[s]) (void)hipHostFree(ctx->pin[s]);
+        ctx->pin[s] = nullptr;
+        ctx->pin_busy[s] = false;
+    }
+}
+
 static int check_finite_host(const double* p, int64_t n) {
     for (int64_t i = 0; i < n; ++i)
         if (!std::isfinite(p[i])) return CIPSI_ERR_NONFINITE;
@@ -404,6 +510,26 @@ static int upload_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* c, int ldc,
 
 double err_estimate(int n, int precision, int n_slices, double vmax);
 
+// Scan the device copy of the caller's values (finiteness, max |x|; up to the first zero
+// label when labels are given) and wait for the result, so a non-finite input is
+// rejected before any transform work.  The scan runs at HBM speed instead of one host
+// core's.
+static int scan_uploaded(cipsi_ijkl_ctx* ctx, const double* d_p, int64_t n,
+                         const int32_t* d_labels, double* vmax) {
+    int rc = ctx_alloc(ctx, ctx->scan, 64);
+    if (rc) return rc;
+    unsigned long long* s = (unsigned long long*)ctx->scan.ptr;
+    HIP_TRY(ctx, launch_scan_input(d_p, n, d_labels, s, ctx->stream));
+    unsigned long long hs[3] = {0ull, 0ull, 0ull};
+    HIP_TRY(ctx, hipMemcpyAsync(hs, s, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (hs[1]) return CIPSI_ERR_NONFINITE;
+    double v;
+    std::memcpy(&v, &hs[0], sizeof(v));
+    *vmax = v;
+    return CIPSI_OK;
+}
+
 int cipsi_ijkl_transform_packed(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* ao,
                                 const double* c, int ldc, double* mo, int precision, int n_slices,
                                 double* err_est) {
@@ -413,22 +539,17 @@ int cipsi_ijkl_transform_packed(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
     if ((rc = check_prec(precision, n_slices))) return rc;
     if (!ao || !c || !mo) return CIPSI_ERR_ARG;
     const int64_t nin = npairs(npairs(n_ao)), nout = npairs(npairs(n_mo));
-    double vmax = 0.0;
-    for (int64_t i = 0; i < nin; ++i) {
-        if (!std::isfinite(ao[i])) return CIPSI_ERR_NONFINITE;
-        vmax = std::fmax(vmax, std::fabs(ao[i]));
-    }
     if ((rc = ctx_alloc(ctx, ctx->io_in, sizeof(double) * nin))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->io_out, sizeof(double) * nout))) return rc;
     double* d_c;
     if ((rc = upload_c(ctx, n_ao, n_mo, c, ldc, &d_c))) return rc;
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->io_in.ptr, ao, sizeof(double) * nin, hipMemcpyHostToDevice,
-                                ctx->stream));
+    if ((rc = copy_h2d(ctx, ctx->io_in.ptr, ao, sizeof(double) * nin))) return rc;
+    double vmax = 0.0;
+    if ((rc = scan_uploaded(ctx, (const double*)ctx->io_in.ptr, nin, nullptr, &vmax))) return rc;
     if ((rc = cipsi_ijkl_transform_packed_dev(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr, d_c,
                                               ldc, (double*)ctx->io_out.ptr, precision, n_slices)))
         return rc;
-    HIP_TRY(ctx, hipMemcpyAsync(mo, ctx->io_out.ptr, sizeof(double) * nout, hipMemcpyDeviceToHost,
-                                ctx->stream));
+    if ((rc = copy_d2h(ctx, mo, ctx->io_out.ptr, sizeof(double) * nout))) return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (err_est) *err_est = err_estimate(n_ao, precision, n_slices, vmax);
     return CIPSI_OK;
@@ -443,12 +564,6 @@ int cipsi_ijkl_transform_labels(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
     if (n_ao > 255) return CIPSI_ERR_LABEL;
     if ((rc = check_prec(precision, n_slices))) return rc;
     if (n_entries < 0 || !c || !mo || (n_entries > 0 && (!xx || !labels))) return CIPSI_ERR_ARG;
-    double vmax = 0.0;
-    for (int64_t i = 0; i < n_entries; ++i) {
-        if (labels[i] == 0) break;
-        if (!std::isfinite(xx[i])) return CIPSI_ERR_NONFINITE;
-        vmax = std::fmax(vmax, 8.0 * std::fabs(xx[i]));
-    }
     const int64_t nout = npairs(npairs(n_mo));
     const int64_t ne = n_entries > 0 ? n_entries : 1;
     if ((rc = ctx_alloc(ctx, ctx->io_in, sizeof(double) * ne))) return rc;
@@ -457,17 +572,19 @@ int cipsi_ijkl_transform_labels(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
     double* d_c;
     if ((rc = upload_c(ctx, n_ao, n_mo, c, ldc, &d_c))) return rc;
     if (n_entries > 0) {
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->io_in.ptr, xx, sizeof(double) * n_entries,
-                                    hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->io_in2.ptr, labels, sizeof(int32_t) * n_entries,
-                                    hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = copy_h2d(ctx, ctx->io_in.ptr, xx, sizeof(double) * n_entries))) return rc;
+        if ((rc = copy_h2d(ctx, ctx->io_in2.ptr, labels, sizeof(int32_t) * n_entries))) return rc;
     }
+    double vmax = 0.0;          // 8 max |xx| before the first zero label (transd's largest scale)
+    if ((rc = scan_uploaded(ctx, (const double*)ctx->io_in.ptr, n_entries,
+                            (const int32_t*)ctx->io_in2.ptr, &vmax)))
+        return rc;
+    vmax *= 8.0;
     if ((rc = cipsi_ijkl_transform_labels_dev(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr,
                                               (const int32_t*)ctx->io_in2.ptr, n_entries, d_c, ldc,
                                               (double*)ctx->io_out.ptr, precision, n_slices)))
         return rc;
-    HIP_TRY(ctx, hipMemcpyAsync(mo, ctx->io_out.ptr, sizeof(double) * nout, hipMemcpyDeviceToHost,
-                                ctx->stream));
+    if ((rc = copy_d2h(ctx, mo, ctx->io_out.ptr, sizeof(double) * nout))) return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (err_est) *err_est = err_estimate(n_ao, precision, n_slices, vmax);
     return CIPSI_OK;

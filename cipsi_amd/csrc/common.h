@@ -79,6 +79,11 @@ hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t
                                 int n_ao, double* ao_packed, uint32_t* owner,
                                 unsigned long long* sc, int* err_flag, hipStream_t st);
 hipError_t launch_expand_packed(const double* ao_packed, int64_t pa, double* vsq, hipStream_t st);
+// Input scan for the host entry points: s[0] = bits of max |x|, s[1] = any non-finite,
+// over p[0 .. end) with end = the first zero label's index (labels given) or n.  p must
+// be 16-byte aligned; s holds three 64-bit words.
+hipError_t launch_scan_input(const double* p, int64_t n, const int32_t* labels,
+                             unsigned long long* s, hipStream_t st);
 hipError_t launch_transpose(const double* src, int64_t rows, int64_t cols, double* dst,
                             hipStream_t st);
 hipError_t launch_pad_c(const double* c, int ldc, int n, int m, int np, int mp, double* cr,

@@ -27,6 +27,14 @@ struct cipsi_ijkl_ctx {
     DevBuf packed;   // packed AO block built from labels
     DevBuf owner;    // last-writer election for duplicate labels
     DevBuf scal;     // small device scalars (first zero label, error flag)
+    DevBuf scan;     // host entry points: input scan result (max |x| bits, non-finite, end)
+    // host entry points: pinned bounce slots for the large transfers (allocated on first
+    // use), each with the event of the last copy that used it
+    static constexpr int kPinSlots = 4;
+    static constexpr size_t kPinBytes = size_t(64) << 20;
+    void* pin[kPinSlots] = {};
+    hipEvent_t pin_ev[kPinSlots] = {};
+    bool pin_busy[kPinSlots] = {};
     DevBuf io_in, io_in2, io_out, io_c;   // staging for host-pointer entry points
     DevBuf tile_start;                    // half-2 work table
     DevBuf slices;   // split-FP16 operand slices

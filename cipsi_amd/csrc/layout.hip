@@ -239,6 +239,77 @@ hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t
     return hipGetLastError();
 }
 
+// ----------------------------------------------------------------------------- input scan
+// Finiteness and max |x| of a device copy of the caller's values, for the host entry
+// points (which reject non-finite input before computing, and use max |x| in the
+// split-FP16 error estimate).  s[0] = bit pattern of max |x| over the finite values (the
+// order of non-negative doubles is the order of their bit patterns), s[1] = 1 if any
+// value is NaN or infinite, s[2] = index of the first zero label (labels given) or n.
+__global__ void k_scan_init(unsigned long long* s, int64_t n) {
+    if (threadIdx.x == 0) { s[0] = 0ull; s[1] = 0ull; s[2] = (unsigned long long)n; }
+}
+
+__global__ __launch_bounds__(256) void k_first_zero(const int32_t* __restrict__ labels, int64_t n,
+                                                    unsigned long long* s) {
+    unsigned long long first = (unsigned long long)n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        if (labels[i] == 0) { first = (unsigned long long)i; break; }   // ascending per thread
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = __shfl_xor(first, o);
+        first = x < first ? x : first;
+    }
+    if ((threadIdx.x & 63) == 0 && first < (unsigned long long)n) atomicMin(&s[2], first);
+}
+
+__global__ __launch_bounds__(256) void k_scan_finite(const double* __restrict__ p, int64_t n,
+                                                     unsigned long long* s) {
+    const int64_t end = (int64_t)s[2] < n ? (int64_t)s[2] : n;      // k_scan_init sets n
+    unsigned long long m = 0ull, bad = 0ull;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n2 = end >> 1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
+        const double2 v = ((const double2*)p)[i];                    // p is 16-byte aligned
+        const double a = fabs(v.x), b = fabs(v.y);
+        if (!isfinite(a) || !isfinite(b)) bad = 1ull;
+        else {
+            const unsigned long long ba = (unsigned long long)__double_as_longlong(a);
+            const unsigned long long bb = (unsigned long long)__double_as_longlong(b);
+            m = ba > m ? ba : m;
+            m = bb > m ? bb : m;
+        }
+    }
+    if ((end & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const double a = fabs(p[end - 1]);
+        if (!isfinite(a)) bad = 1ull;
+        else {
+            const unsigned long long ba = (unsigned long long)__double_as_longlong(a);
+            m = ba > m ? ba : m;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = __shfl_xor(m, o);
+        m = x > m ? x : m;
+        bad |= __shfl_xor(bad, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (m) atomicMax(&s[0], m);
+        if (bad) atomicOr(&s[1], 1ull);
+    }
+}
+
+hipError_t launch_scan_input(const double* p, int64_t n, const int32_t* labels,
+                             unsigned long long* s, hipStream_t st) {
+    if (n < 0 || ((uintptr_t)p & 15)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scan_init, dim3(1), dim3(64), 0, st, s, n);
+    if (n == 0) return hipGetLastError();
+    if (labels)
+        hipLaunchKernelGGL(k_first_zero, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, labels, n, s);
+    hipLaunchKernelGGL(k_scan_finite, dim3(grid_for((n + 1) / 2, 256, 8192)), dim3(256), 0, st, p, n,
+                       s);
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------------------------- expand
 // The packed AO block stores each unordered pair-of-pairs once; the sandwich kernel
 // wants row pq of the symmetric pair matrix contiguous.  64 x 64 tiles over the

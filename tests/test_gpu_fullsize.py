@@ -119,3 +119,24 @@ def test_every_row_class_against_closed_form(ctx, n, m):
     o = out.cpu().numpy()
     idx = np.concatenate([np.arange(k * (k + 1) // 2, k * (k + 1) // 2 + k + 1) for k in kl_list])
     assert np.max(np.abs(o_ref[idx] - o[idx])) <= 1e-12
+
+
+def test_host_entry_points_over_many_bounce_chunks(ctx):
+    """At N_AO = N_MO = 140 the packed blocks are 390 MB, seven 64 MiB chunks of the
+    host entry points' pinned bounce slots (four slots, so each is reused), and the
+    label stream is 390 MB of values plus 195 MB of labels.  The host entry points
+    (packed and label stream) give the device path's bits, and the label stream is the
+    pshf stream of the same block."""
+    from cipsi_amd import formats
+    n = m = 140
+    print(f"[bounce] n={n}", flush=True)
+    B = synth.eri_factors(n, 2 * n, 10.0, 77)
+    ao = synth.eri_packed_torch(B, "cuda")
+    c = synth.orthonormal(n, 78)[:, :m]
+    c_dev = torch.as_tensor(np.ascontiguousarray(c), device="cuda")
+    ref = ctx.transform(ao, c_dev).cpu().numpy()
+    ao_np = ao.cpu().numpy()
+    assert np.array_equal(ctx.transform(ao_np, c), ref)
+    xx, ix = formats.labels_from_packed(ao_np, n)
+    assert xx.nbytes > 4 * (64 << 20)
+    assert np.array_equal(ctx.itijkl(xx, ix, c), ref)

@@ -221,3 +221,38 @@ def test_transform_under_graph_capture(ctx):
             g.replay()
             torch.cuda.synchronize()
             assert torch.equal(out, refs[k]), f"replay {rep} case {k}"
+
+
+def test_host_entry_input_scan(ctx):
+    """The host entry points scan the uploaded values on the device before computing: a
+    NaN or an infinity anywhere (the odd last element too) is rejected with
+    CIPSI_ERR_NONFINITE; in a label stream only the entries before the first zero label
+    count (itijkl.f:62-68), so a NaN in the stale tail after it is accepted.  The
+    split-FP16 error estimate scales with max |x|, so doubling the input doubles it."""
+    from cipsi_amd import ijkl
+    n, m = 6, 5                                  # 231 packed values: odd length
+    case = synth.make_case(n, m, 2 * n, 3.0, 41)
+    V, c = case["ao_packed"], case["c"]
+    assert V.size % 2 == 1
+    ref = oracle.ijkl(V, c)
+    for pos, val in [(V.size - 1, np.inf), (0, np.nan), (V.size // 2, -np.inf)]:
+        bad = V.copy()
+        bad[pos] = val
+        with pytest.raises(ijkl.IjklError) as e:
+            ctx.transform(bad, c)
+        assert e.value.status == -5, (pos, val)
+    assert np.max(np.abs(ctx.transform(V, c) - ref)) <= TOL
+    xx, ix = formats.labels_from_packed(V, n)
+    tail_x = np.array([0.0, np.nan, 1.0])          # terminator, then a stale tail
+    tail_i = np.array([0, formats.label_encode(2, 1, 1, 1), formats.label_encode(1, 1, 1, 1)],
+                      dtype=np.int32)
+    out = ctx.itijkl(np.concatenate([xx, tail_x]), np.concatenate([ix, tail_i]), c)
+    assert np.max(np.abs(out - ref)) <= TOL
+    bad_x = xx.copy()
+    bad_x[-1] = np.nan                              # last entry before the terminator
+    with pytest.raises(ijkl.IjklError) as e:
+        ctx.itijkl(np.concatenate([bad_x, tail_x]), np.concatenate([ix, tail_i]), c)
+    assert e.value.status == -5
+    _, e1 = ctx.transform(V, c, precision="split_fp16", return_err=True)
+    _, e2 = ctx.transform(2.0 * V, c, precision="split_fp16", return_err=True)
+    assert e1 > 0 and e2 == pytest.approx(2.0 * e1, rel=1e-12)
