@@ -21,11 +21,13 @@ autotuned tile variants per row class; reports geometries/s, the FP64 MFMA flop 
 of the whole launch against the matrix peak, and the same scan run as a loop of
 single transforms for comparison.
 
-Label ingest (--workload labels) and integral consumers (--workload consumers: the ai()
-gather and the Fock build on the transformed stream) print lines of their own.
+Label ingest (--workload labels), integral consumers (--workload consumers: the ai()
+gather, the Fock build and the Slater-Condon elements on the transformed stream) and the
+adjoint (--workload adjoint: E and dE/dC at N_AO = 160, N_MO = 100) print lines of their
+own.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|split_fp16|both]
-                       [--workload transform|scan|labels|consumers]
+                       [--workload transform|scan|labels|consumers|adjoint]
 """
 from __future__ import annotations
 
@@ -267,6 +269,62 @@ def run_labels(args, torch):
     }))
 
 
+def run_adjoint(args, torch):
+    """Energy and adjoint gradient E(C), dE/dC at N_AO = 160, N_MO = 100 (rank 480, decay
+    8; seeded h, gamma and a PSD trace-1 pair density Gamma): the forward transform is
+    recomputed, E is reduced against Gamma and gamma, and the gradient takes one
+    P_A x P_M x P_M DGEMM (L = H Gamma_hat) plus the contraction with C."""
+    from cipsi_amd import ijkl, synth
+    n, m = 160, 100
+    B = synth.eri_factors(n, 3 * n, 8.0, 2028)
+    ao = synth.eri_packed_torch(B, "cuda")
+    rng = np.random.default_rng(2035)
+    c = np.asfortranarray(synth.orthonormal(n, 2029)[:, :m] + 0.05 * rng.standard_normal((n, m)))
+    h = torch.as_tensor(synth.symmetric_packed(n, 2031), device="cuda")
+    gamma, Gam = synth.pair_density(m, 2033)
+    gamma = torch.as_tensor(np.ascontiguousarray(gamma).ravel(), device="cuda")
+    Gam = torch.as_tensor(Gam, device="cuda")
+    c_dev = torch.as_tensor(np.ascontiguousarray(c), device="cuda")
+    ctx = ijkl.Context(0)
+    for _ in range(args.warmup):
+        ctx.energy_grad(ao, h, c_dev, gamma, Gam)
+    torch.cuda.synchronize()
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e, g = ctx.energy_grad(ao, h, c_dev, gamma, Gam)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ctx.enable_timing(False)
+    stages = {}
+    for st in ("pad_c", "expand", "half1", "transpose", "half2", "energy", "grad_gemm",
+               "grad_contract"):
+        tot, cnt = ctx.read_timing(st)
+        stages[st] = round(tot / cnt, 4) if cnt else 0.0
+    pa, pm = n * (n + 1) // 2, m * (m + 1) // 2
+    gemm_flops = 2.0 * pm * pa * pm
+    gemm_tf = gemm_flops / (stages["grad_gemm"] * 1e-3) / 1e12 if stages["grad_gemm"] else 0.0
+    fwd_flops = flops_half1(n, m) + flops_half2(n, m)
+    result = {
+        "metric": "energy+gradient evaluations/s (N_AO=160, N_MO=100)", "value": 1.0 / dt,
+        "unit": "evaluations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded ERI factors, perturbed orthonormal C, h, gamma, Gamma)",
+        "config": {"workload": "adjoint dE/dC, N_AO=160 N_MO=100 rank=480 decay=8"},
+        "energy": float(e), "grad_norm": float(g.norm()),
+        "stage_ms": stages,
+        "roofline": {"bound": "mfma", "kernel": "rocBLAS DGEMM L = H Gamma_hat (grad_gemm stage)",
+                     "achieved": round(gemm_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(gemm_tf / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "flops": gemm_flops},
+        "forward_tflops": round(fwd_flops / ((stages["half1"] + stages["half2"]) * 1e-3) / 1e12, 3)
+        if stages["half1"] else None,
+    }
+    print(json.dumps(result))
+
+
 def run_consumers(args, torch):
     """Integral consumers on the device after one N_MO = 240 transform: the batched ai()
     gather over 2^27 uniformly random labels (HBM random access: 4 B label + 8 B
@@ -483,7 +541,8 @@ def main():
     ap.add_argument("--n-mo", type=int, default=240)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--cpu-threads", type=int, default=1)
-    ap.add_argument("--workload", default="transform", choices=["transform", "scan", "labels", "consumers"])
+    ap.add_argument("--workload", default="transform",
+                    choices=["transform", "scan", "labels", "consumers", "adjoint"])
     ap.add_argument("--scan-variants", default="",
                     help='scan: fixed tile variants per row class "v1,v2;v1,v2;..." (no autotune)')
     ap.add_argument("--no-loop", action="store_true",
@@ -508,6 +567,9 @@ def main():
         return
     if args.workload == "consumers":
         run_consumers(args, torch)
+        return
+    if args.workload == "adjoint":
+        run_adjoint(args, torch)
         return
 
     n, m = args.n_ao, args.n_mo

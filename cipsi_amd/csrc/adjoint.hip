@@ -122,30 +122,58 @@ __global__ void k_hc(const double* __restrict__ hp, const double* __restrict__ c
 }
 
 // grad(mu, p) = 2 sum_q hc(mu,q) gamma(q,p) + 2 sum_{nu,q} C(nu,q) L[pair(mu,nu)][pair(p,q)]
-// One work-group per (mu, p); the (nu, q) sum is split over threads with q fastest, so
-// that for q <= p a wave reads a contiguous piece of row pair(mu,nu) of L.
-__global__ __launch_bounds__(256) void k_grad(const double* __restrict__ L, int64_t pm,
-                                              const double* __restrict__ c, int ldc,
-                                              const double* __restrict__ hc,
-                                              const double* __restrict__ gamma, int n, int m,
-                                              double* __restrict__ grad) {
-    __shared__ double red[256];
-    const int mu = blockIdx.x;
-    const int p = blockIdx.y;
-    const int tid = threadIdx.x;
-    double s = 0.0;
-    for (int e = tid; e < n * m; e += 256) {
-        const int nu = e / m, q = e - nu * m;
-        s += c[(int64_t)q * ldc + nu] * L[pairsym(mu, nu) * pm + pairsym(p, q)];
-    }
-    for (int q = tid; q < m; q += 256) s += hc[(int64_t)q * n + mu] * gamma[(int64_t)q * m + p];
-    red[tid] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (tid < off) red[tid] += red[tid + off];
+// Row r = pair(mu, nu) of L, read as the symmetric M x M matrix M_r(p, q) =
+// L[r][pair(p,q)], enters grad(mu, .) as M_r C(nu, .)^T.  One work-group per (mu, chunk
+// of kGradNu values of nu) streams those rows once each: thread p forms sum_q M_r(p,q)
+// C(nu,q), reading its own contiguous piece (q <= p) and then, coalesced with the other
+// threads, the column below the diagonal (q > p).  C(nu, .) is staged in LDS.  Partial
+// sums per chunk go to part[chunk][mu][p]; k_grad_sum adds them in chunk order (so the
+// result is the same on every run) together with the one-electron term.
+constexpr int kGradNu = 8;
+
+__global__ __launch_bounds__(256) void k_grad_rows(const double* __restrict__ L, int64_t pm,
+                                                   const double* __restrict__ c, int ldc, int n,
+                                                   int m, double* __restrict__ part) {
+    __shared__ double cs[256];
+    const int mu = blockIdx.x, ch = blockIdx.y;
+    const int p = threadIdx.x;
+    const int nu0 = ch * kGradNu, nu1 = min(n, nu0 + kGradNu);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};               // q mod 4: independent FMA chains
+    for (int nu = nu0; nu < nu1; ++nu) {
+        __syncthreads();                               // previous nu's reads of cs done
+        if (p < m) cs[p] = c[(int64_t)p * ldc + nu];
         __syncthreads();
+        if (p < m) {
+            const double* __restrict__ row = L + pairsym(mu, nu) * pm;
+            const double* __restrict__ rp = row + (int64_t)p * (p + 1) / 2;
+            int q = 0;
+            for (; q + 3 <= p; q += 4) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[e] += rp[q + e] * cs[q + e];
+            }
+            for (; q <= p; ++q) acc[q & 3] += rp[q] * cs[q];
+            for (q = p + 1; q + 3 < m; q += 4) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    acc[(q + e) & 3] += row[(int64_t)(q + e) * (q + e + 1) / 2 + p] * cs[q + e];
+            }
+            for (; q < m; ++q) acc[q & 3] += row[(int64_t)q * (q + 1) / 2 + p] * cs[q];
+        }
     }
-    if (tid == 0) grad[(int64_t)p * n + mu] = 2.0 * red[0];
+    if (p < m) part[((int64_t)ch * n + mu) * m + p] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+__global__ __launch_bounds__(256) void k_grad_sum(const double* __restrict__ part, int nch,
+                                                  const double* __restrict__ hc,
+                                                  const double* __restrict__ gamma, int n, int m,
+                                                  double* __restrict__ grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * m) return;
+    const int mu = i / m, p = i - mu * m;
+    double s = 0.0;
+    for (int ch = 0; ch < nch; ++ch) s += part[((int64_t)ch * n + mu) * m + p];
+    for (int q = 0; q < m; ++q) s += hc[(int64_t)q * n + mu] * gamma[(int64_t)q * m + p];
+    grad[(int64_t)p * n + mu] = 2.0 * s;
 }
 
 }  // namespace
@@ -160,14 +188,17 @@ static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, co
     if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->h, sizeof(double) * pa * pm))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->adj_g, sizeof(double) * nout))) return rc;
-    if ((rc = ctx_alloc(ctx, ctx->adj_small, sizeof(double) * (pm + 2 * (size_t)n * m + 4096))))
+    const int nch = (n + kGradNu - 1) / kGradNu;
+    if ((rc = ctx_alloc(ctx, ctx->adj_small,
+                        sizeof(double) * (pm + (2 + (size_t)nch) * n * m + 4096))))
         return rc;
     double* vsq = (double*)ctx->vsq.ptr;
     double* h = (double*)ctx->h.ptr;
     double* g = (double*)ctx->adj_g.ptr;
     double* hmo = (double*)ctx->adj_small.ptr;
     double* hc = hmo + pm;
-    double* part = hc + (size_t)n * m;
+    double* part = hc + (size_t)n * m;                // energy partials (4096), then grad
+    double* gpart = part + 4096;                       // partials (nch x n x m)
     double* cr;
     if ((rc = run_pad_c(ctx, n, m, d_c, ldc, &cr))) return rc;
     {
@@ -228,8 +259,10 @@ static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, co
     StageTimer tm(ctx, "grad_contract");
     hipLaunchKernelGGL(k_hc, dim3((n * m + 255) / 256), dim3(256), 0, ctx->stream, d_hp, d_c, ldc,
                        n, m, hc);
-    hipLaunchKernelGGL(k_grad, dim3(n, m), dim3(256), 0, ctx->stream, L, pm, d_c, ldc, hc, d_gamma,
-                       n, m, d_grad);
+    hipLaunchKernelGGL(k_grad_rows, dim3(n, nch), dim3(256), 0, ctx->stream, L, pm, d_c, ldc, n, m,
+                       gpart);
+    hipLaunchKernelGGL(k_grad_sum, dim3((n * m + 255) / 256), dim3(256), 0, ctx->stream, gpart,
+                       nch, hc, d_gamma, n, m, d_grad);
     return hipGetLastError() == hipSuccess ? CIPSI_OK : CIPSI_ERR_HIP;
 }
 
