@@ -407,7 +407,12 @@ static void parallel_memcpy(void* dst, const void* src, size_t bytes) {
     int used = 0;
     for (size_t off = 0; off < bytes; off += part) {
         const size_t len = std::min(part, bytes - off);
-        th[used++] = std::thread([=] { std::memcpy((char*)dst + off, (const char*)src + off, len); });
+        try {           // no exception may leave the C ABI: copy here if no thread starts
+            th[used] = std::thread([=] { std::memcpy((char*)dst + off, (const char*)src + off, len); });
+            ++used;
+        } catch (...) {
+            std::memcpy((char*)dst + off, (const char*)src + off, len);
+        }
     }
     for (int t = 0; t < used; ++t) th[t].join();
 }
