@@ -75,6 +75,10 @@ int cipsi_ijkl_variant_info(int v, int* waves, int* kc, int* minb) {
 
 int cipsi_ijkl_set_plan(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, int precision, int v1, int v2) {
     if (!ctx || v1 >= kNumHalfVariants || v2 >= kNumHalfVariants) return CIPSI_ERR_ARG;
+    const int rc_cls = row_class((n_ao + 15) & ~15);
+    if (precision == CIPSI_PREC_FP64 &&
+        ((v1 >= 0 && !variant_supports(v1, rc_cls)) || (v2 >= 0 && !variant_supports(v2, rc_cls))))
+        return CIPSI_ERR_UNSUPPORTED;
     auto key = std::make_tuple(n_ao, n_mo, precision);
     auto it = ctx->plans.find(key);
     std::pair<int, int> cur = it != ctx->plans.end() ? it->second
@@ -112,11 +116,14 @@ int cipsi_ijkl_autotune(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, int precision, 
     const double* cr = (const double*)ctx->cr.ptr;
     int v1 = ctx->var_half1, v2 = ctx->var_half2;
     double t1 = DBL_MAX, t2 = DBL_MAX;
+    const int rcls = row_class((n_ao + 15) & ~15);
     for (int v = 0; v < kNumHalfVariants; ++v) {
+        if (!variant_supports(v, rcls)) continue;
         double t = time_variant(ctx, 1, n_ao, n_mo, vsq, cr, h, precision, v);
         if (t < t1) { t1 = t; v1 = v; }
     }
     for (int v = 0; v < kNumHalfVariants; ++v) {
+        if (!variant_supports(v, rcls)) continue;
         double t = time_variant(ctx, 2, n_ao, n_mo, vsq, cr, (double*)ctx->io_out.ptr, precision, v);
         if (t < t2) { t2 = t; v2 = v; }
     }

@@ -25,9 +25,9 @@ using namespace cipsi;
 
 namespace {
 
-constexpr int kLts[3] = {32, 64, 128};            // columns per work-group of the variants
+constexpr int kLts[4] = {32, 64, 128, 192};       // columns per work-group of the variants
 
-int lt_index(int lt) { return lt == 32 ? 0 : (lt == 64 ? 1 : 2); }
+int lt_index(int lt) { return lt == 32 ? 0 : (lt == 64 ? 1 : (lt == 128 ? 2 : 3)); }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -54,7 +54,7 @@ struct cipsi_ijkl_batch {
     std::vector<size_t> off_vsq, off_h, off_cr;
     void* tables = nullptr;      // LayoutGeom[ngeom] | HalfArgs h1[ngeom] | HalfArgs h2[ngeom]
                                  // | start1[ngeom + ncls] | start2[ngeom + ncls]
-    void* tile_tables = nullptr; // half-2 tile prefix tables, 3 column widths x sum(m + 1)
+    void* tile_tables = nullptr; // half-2 tile prefix tables, 4 column widths x sum(m + 1)
     std::vector<int64_t> tt_off; // offset of geometry g's table (per width: + w * tt_stride)
     int64_t tt_stride = 0;
     size_t tables_bytes = 0;
@@ -373,15 +373,15 @@ int cipsi_ijkl_batch_create(cipsi_ijkl_ctx* ctx, int n_geom, const int* n_ao, co
     b->off_s2 = t;
     t = align256(t + sizeof(int64_t) * (n_geom + ncls));
     b->tables_bytes = t;
-    // half-2 tile prefix tables for the three column widths
+    // half-2 tile prefix tables for the four column widths
     int64_t tts = 0;
     for (int g = 0; g < n_geom; ++g) {
         b->tt_off.push_back(tts);
         tts += b->m[g] + 1;
     }
     b->tt_stride = tts;
-    std::vector<int64_t> tt(3 * tts);
-    for (int w = 0; w < 3; ++w)
+    std::vector<int64_t> tt(4 * tts);
+    for (int w = 0; w < 4; ++w)
         for (int g = 0; g < n_geom; ++g) {
             int64_t* p = tt.data() + w * tts + b->tt_off[g];
             p[0] = 0;
@@ -458,6 +458,7 @@ int cipsi_ijkl_batch_autotune(cipsi_ijkl_ctx* ctx, cipsi_ijkl_batch* b, const do
             int best_v = half == 1 ? cp.v1 : cp.v2;
             float best = 1e30f;
             for (int v = 0; v < kNumHalfVariants; ++v) {
+                if (!variant_supports(v, cp.row_class)) continue;
                 (half == 1 ? cp.v1 : cp.v2) = v;
                 if ((rc = prepare_class(ctx, cp))) return rc;
                 if ((rc = upload_tables(ctx, b, d_ao, d_c, ldc, d_mo))) return rc;
@@ -506,6 +507,8 @@ int cipsi_ijkl_batch_set_class_plan(cipsi_ijkl_ctx* ctx, cipsi_ijkl_batch* b, in
         var_half1 >= kNumHalfVariants || var_half2 < 0 || var_half2 >= kNumHalfVariants)
         return CIPSI_ERR_ARG;
     ClassPlan& cp = b->classes[k];
+    if (!variant_supports(var_half1, cp.row_class) || !variant_supports(var_half2, cp.row_class))
+        return CIPSI_ERR_UNSUPPORTED;
     cp.v1 = var_half1;
     cp.v2 = var_half2;
     drop_graph(b);

@@ -118,6 +118,8 @@ int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, con
 
 int run_half_variant(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src,
                      const double* cr, double* dst, int precision, int n_slices, int var) {
+    if (precision == CIPSI_PREC_FP64 && !variant_supports(var, row_class(round16(n))))
+        return CIPSI_ERR_UNSUPPORTED;
     const int64_t pa = npairs(n), pm = npairs(m);
     HalfArgs a{};
     a.src = src;

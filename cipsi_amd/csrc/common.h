@@ -73,6 +73,8 @@ struct HalfVariant {
 extern const HalfVariant kHalfVariants[];
 extern const int kNumHalfVariants;
 inline int variant_columns(int v) { return 16 * kHalfVariants[v].waves; }
+// Whether variant v is built for a row class (the number of 16-row blocks of N_AO).
+bool variant_supports(int v, int row_cls);
 
 // Device launchers (defined in the .hip files).
 hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t n_entries,

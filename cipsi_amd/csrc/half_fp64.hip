@@ -337,11 +337,20 @@ static hipError_t launch_group_v(const GroupLaunch& gl, hipStream_t st) {
 }
 
 // Tile variants: (waves per work-group, Q1 panel depth, work-groups per CU hint).
-// Columns per work-group = 16 x waves.
+// Columns per work-group = 16 x waves.  The 12-wave variant runs three waves per SIMD,
+// which caps a wave at 170 registers: its T accumulators fit for up to 12 row blocks
+// (N_AO <= 192); with more it would spill, so it is not built for those row classes.
 const HalfVariant kHalfVariants[] = {
     {4, 16, 1}, {4, 32, 1}, {8, 16, 1}, {8, 32, 1}, {2, 16, 1}, {4, 16, 2}, {8, 8, 1},
+    {12, 16, 1},
 };
 const int kNumHalfVariants = sizeof(kHalfVariants) / sizeof(kHalfVariants[0]);
+constexpr int kMaxRows12 = 12;
+
+bool variant_supports(int v, int row_cls) {
+    if (v < 0 || v >= kNumHalfVariants) return false;
+    return kHalfVariants[v].waves != 12 || row_cls <= kMaxRows12;
+}
 
 template <int NRB>
 static hipError_t launch_rows(const HalfArgs& a, int v, hipStream_t st) {
@@ -353,6 +362,9 @@ static hipError_t launch_rows(const HalfArgs& a, int v, hipStream_t st) {
         case 4: return launch_v<2, NRB, 16, 1>(a, st);
         case 5: return launch_v<4, NRB, 16, 2>(a, st);
         case 6: return launch_v<8, NRB, 8, 1>(a, st);
+        case 7:
+            if constexpr (NRB <= kMaxRows12) return launch_v<12, NRB, 16, 1>(a, st);
+            else return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }
@@ -367,6 +379,9 @@ static hipError_t launch_group_rows(const GroupLaunch& gl, hipStream_t st) {
         case 4: return launch_group_v<2, NRB, 16, 1>(gl, st);
         case 5: return launch_group_v<4, NRB, 16, 2>(gl, st);
         case 6: return launch_group_v<8, NRB, 8, 1>(gl, st);
+        case 7:
+            if constexpr (NRB <= kMaxRows12) return launch_group_v<12, NRB, 16, 1>(gl, st);
+            else return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }

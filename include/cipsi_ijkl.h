@@ -41,7 +41,7 @@ extern "C" {
 #define CIPSI_ERR_HIP            -4   /* other HIP runtime failure */
 #define CIPSI_ERR_NONFINITE      -5   /* NaN or Inf in an input */
 #define CIPSI_ERR_NOT_ORTHONORMAL -6  /* |C^T S C - 1| above tolerance */
-#define CIPSI_ERR_UNSUPPORTED    -7   /* n_ao > 256 or a precision / slice count not built */
+#define CIPSI_ERR_UNSUPPORTED    -7   /* n_ao > 256, or a precision / slice count / tile variant not built */
 #define CIPSI_ERR_NODEVICE       -8   /* no usable gfx950 device */
 
 /* precision argument */
@@ -145,7 +145,10 @@ int cipsi_ijkl_energy_grad_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo,
    reference; tile shapes are specific to this implementation. */
 int cipsi_ijkl_autotune(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, int precision,
                         int *best_half1, int *best_half2);
-/* Set / read the plan (variant indices) used for a size; -1 leaves a half unchanged. */
+/* Set / read the plan (variant indices) used for a size; -1 leaves a half unchanged.
+   A 12-wave variant is built only for N_AO <= 192: set_plan (and
+   cipsi_ijkl_batch_set_class_plan for a row class above 12 blocks of 16) returns
+   CIPSI_ERR_UNSUPPORTED for it, and the autotuners skip it there. */
 int cipsi_ijkl_set_plan(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, int precision, int var_half1,
                         int var_half2);
 int cipsi_ijkl_get_plan(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, int precision, int *var_half1,
