@@ -205,7 +205,7 @@ GroupLaunch group_args(const cipsi_ijkl_batch* b, size_t k, int half) {
     gl.row_class = cp.row_class;
     gl.variant = half == 1 ? cp.v1 : cp.v2;
     const int64_t items = half == 1 ? cp.items1 : cp.items2;
-    gl.grid = (items + 7) / 8 * 8;
+    gl.grid = xcd_grid(items);
     return gl;
 }
 

@@ -107,6 +107,22 @@ struct GroupLaunch {
     int64_t grid;    // > 0: launch with this grid (>= item total); <= 0: prepare only
 };
 hipError_t launch_half_fp64_group(const GroupLaunch& gl, hipStream_t st);
+// XCD-aware work-item mapping of the half-transform kernels.  The hardware deals
+// work-groups to the 8 XCDs round robin (work-group i runs on XCD i % 8).  Work items
+// are cut into chunks of kXcdChunk consecutive items and the chunks are dealt to the
+// XCDs round robin too: the tiles of one matrix (consecutive items) share an L2, and
+// every XCD gets the same mix of cheap and expensive items (in half 2 an item's cost
+// grows with its MO pair row; contiguous eighths left the last XCD 10-17 % more work
+// than the mean).  xcd_grid rounds a launch up to a multiple of 8 kXcdChunk work-groups,
+// which makes xcd_item a permutation of [0, grid).
+constexpr int kXcdChunk = 16;
+
+__device__ __forceinline__ int64_t xcd_item(unsigned bid) {
+    const int64_t x = bid & 7, j = bid >> 3;
+    return ((j / kXcdChunk) * 8 + x) * kXcdChunk + (j % kXcdChunk);
+}
+
+int64_t xcd_grid(int64_t items);
 
 // Grouped layout kernels over several transforms (one launch each).
 struct LayoutGeom {

@@ -265,12 +265,14 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
     }
 }
 
+int64_t xcd_grid(int64_t items) {
+    constexpr int64_t q = 8 * kXcdChunk;
+    return (items + q - 1) / q * q;
+}
+
 template <int NW, int NRB, int KC, int MINB>
 __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
-    // XCD-aware mapping: consecutive work items (same matrix, neighbouring tiles)
-    // land on the same XCD so the Z_b row is fetched into one L2.
-    const int64_t per_xcd = gridDim.x >> 3;
-    const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int64_t item = xcd_item(blockIdx.x);
     if (item >= A.ntile_total) return;
     half_tile<NW, NRB, KC>(A, item);
 }
@@ -278,17 +280,15 @@ __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
 // Grouped launch over the work items of several transforms (a scan of geometries).
 // Items are numbered geometry after geometry (item_start[g] is the first item of
 // geometry g, item_start[ngeom] the total); the grid holds one work-group per item, so
-// the hardware dispatcher balances the work as in a single transform's launch, and the
-// item order is cut into 8 contiguous pieces, one per XCD (work-group w runs on XCD
-// w % 8), so neighbouring tiles of one matrix share an L2.  The owning geometry is
-// found by binary search over the start table.
+// the hardware dispatcher balances the work as in a single transform's launch, with the
+// same chunked XCD mapping.  The owning geometry is found by binary search over the
+// start table.
 template <int NW, int NRB, int KC, int MINB>
 __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64_group(const HalfArgs* __restrict__ geoms,
                                                                   const int64_t* __restrict__ item_start,
                                                                   int ngeom) {
     const int64_t total = item_start[ngeom];
-    const int64_t per_xcd = gridDim.x >> 3;
-    const int64_t it = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int64_t it = xcd_item(blockIdx.x);
     if (it >= total) return;
     int lo = 0, hi = ngeom;                        // item_start[0] = 0 <= it < total
     while (hi - lo > 1) {
@@ -315,14 +315,14 @@ static hipError_t launch_v(const HalfArgs& a, hipStream_t st) {
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    int64_t g = (a.ntile_total + 7) / 8 * 8;
+    const int64_t g = xcd_grid(a.ntile_total);
     if (g > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_half_fp64<NW, NRB, KC, MINB>), dim3((unsigned)g), dim3(K::NT), shmem, st, a);
     return hipGetLastError();
 }
 
 // Grouped launch.  grid <= 0 only prepares the kernel (LDS attribute); grid > 0
-// launches (one work-group per item, rounded up to a multiple of 8).
+// launches (one work-group per item, rounded up by xcd_grid).
 template <int NW, int NRB, int KC, int MINB>
 static hipError_t launch_group_v(const GroupLaunch& gl, hipStream_t st) {
     using K = HalfCfg<NW, NRB, KC>;
@@ -330,7 +330,7 @@ static hipError_t launch_group_v(const GroupLaunch& gl, hipStream_t st) {
     const size_t shmem = half_shmem<NW, NRB, KC>();
     if (gl.grid <= 0)
         return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (gl.grid > 0x7fffffff || (gl.grid & 7)) return hipErrorInvalidValue;
+    if (gl.grid > 0x7fffffff || gl.grid != xcd_grid(gl.grid)) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_half_fp64_group<NW, NRB, KC, MINB>), dim3((unsigned)gl.grid), dim3(K::NT),
                        shmem, st, gl.geoms, gl.item_start, gl.ngeom);
     return hipGetLastError();

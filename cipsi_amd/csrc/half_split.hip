@@ -148,8 +148,7 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int lm = lane & 15, lg = lane >> 4;
-    const int64_t per_xcd = gridDim.x >> 3;
-    const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int64_t item = xcd_item(blockIdx.x);
     if (item >= A.nitems) return;
 
     // ---- work item -> matrix b, row tile, extents
@@ -360,7 +359,7 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
 
 template <int S, int STEP>
 hipError_t launch_split_gemm(const SplitArgs& a, hipStream_t st) {
-    const int64_t g = (a.nitems + 7) / 8 * 8;
+    const int64_t g = xcd_grid(a.nitems);
     if (g <= 0 || g > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_split_gemm<S, STEP>), dim3((unsigned)g), dim3(NTHR), 0, st, a);
     return hipGetLastError();
