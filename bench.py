@@ -48,7 +48,7 @@ FP64_PEAK_TFLOPS = 78.6
 # WRITE_SIZE runs of this workload (fetched = 2.0 x FETCH_SIZE, written = 1.0 x
 # WRITE_SIZE, factors from a 1 GiB copy in the same process); keyed by
 # (n_ao, n_mo, precision).  Summaries: profiles/fp64_n240_kernels.md.
-MEASURED_TRAFFIC = {(240, 240, "fp64"): (2.0 * 3765500.0 + 6834100.0) * 1024}
+MEASURED_TRAFFIC = {(240, 240, "fp64"): (2.0 * 3631400.0 + 6877300.0) * 1024}
 # k_gather over 2^27 uniformly random labels at N_MO = 240, per launch: 2.0 x FETCH_SIZE
 # 8.6412e6 KiB + 1.0 x WRITE_SIZE 1.0486e6 KiB (profiles/consumers_n240.md).
 MEASURED_TRAFFIC_GATHER = {(240, 1 << 27): (2.0 * 8.6412e6 + 1.0486e6) * 1024}

@@ -112,8 +112,8 @@ hipError_t launch_half_fp64_group(const GroupLaunch& gl, hipStream_t st);
 // are cut into chunks of kXcdChunk consecutive items and the chunks are dealt to the
 // XCDs round robin too: the tiles of one matrix (consecutive items) share an L2, and
 // every XCD gets the same mix of cheap and expensive items (in half 2 an item's cost
-// grows with its MO pair row; contiguous eighths left the last XCD 10-17 % more work
-// than the mean).  xcd_grid rounds a launch up to a multiple of 8 kXcdChunk work-groups,
+// grows with its MO pair row; with contiguous eighths a cost model of the items gives
+// the last XCD 10-17 % more work than the mean).  xcd_grid rounds a launch up to a multiple of 8 kXcdChunk work-groups,
 // which makes xcd_item a permutation of [0, grid).
 constexpr int kXcdChunk = 16;
 
