@@ -265,3 +265,29 @@ def test_host_entry_input_scan(ctx):
     _, e1 = ctx.transform(V, c, precision="split_fp16", return_err=True)
     _, e2 = ctx.transform(2.0 * V, c, precision="split_fp16", return_err=True)
     assert e1 > 0 and e2 == pytest.approx(2.0 * e1, rel=1e-12)
+
+
+@pytest.mark.parametrize("n,m", [(97, 90), (200, 177)])
+def test_autotune_keeps_the_bits_and_built_variants(ctx, n, m):
+    """cipsi_ijkl_autotune times the built variants of both halves on synthetic data of
+    the size and stores the winners as the size's plan: the plan it returns is the one
+    get_plan reads, it never names the 12-wave variant above 12 row blocks (N_AO 200),
+    and the tuned transform gives the default plan's bits."""
+    from cipsi_amd import ijkl
+    print(f"[autotune] n={n} m={m}", flush=True)
+    case = synth.make_case(n, m, 2 * n, 6.0, 900 + n)
+    V, c = dev(case["ao_packed"]), dev(case["c"])
+    ref = ctx.transform(V, c)
+    torch.cuda.synchronize()
+    shapes = ijkl.variants()
+    try:
+        v1, v2 = ctx.autotune(n, m)
+        assert (v1, v2) == ctx.get_plan(n, m)
+        for v in (v1, v2):
+            assert 0 <= v < len(shapes)
+            assert shapes[v][0] != 12 or (n + 15) // 16 <= 12
+        out = ctx.transform(V, c)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref)
+    finally:
+        ctx.set_plan(n, m, *ctx.get_plan(1, 1))
