@@ -13,6 +13,7 @@
 // class by timing the grouped launches on the plan's own data.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -482,6 +483,31 @@ int cipsi_ijkl_batch_autotune(cipsi_ijkl_ctx* ctx, cipsi_ijkl_batch* b, const do
     if ((e = enqueue_all(b, ctx->stream)) != hipSuccess) return fail(ctx, e, "batch autotune");
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(ctx, e, "batch autotune");
     return CIPSI_OK;
+}
+
+// One-call form: the context keeps the plan of its last size list.  A new size list
+// builds a plan and tunes it on this call's data (the autotuner leaves the outputs of a
+// full run); the same size list again replays the captured graph.
+int cipsi_ijkl_transform_batch_dev(cipsi_ijkl_ctx* ctx, int n_geom, const int* n_ao,
+                                   const int* n_mo, const double* const* d_ao,
+                                   const double* const* d_c, const int* ldc,
+                                   double* const* d_mo, int precision, int n_slices) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if (precision == CIPSI_PREC_SPLIT_FP16) return CIPSI_ERR_UNSUPPORTED;
+    if (precision != CIPSI_PREC_FP64 || n_slices != 0) return CIPSI_ERR_ARG;
+    if ((rc = check_sizes(n_geom, n_ao, n_mo))) return rc;
+    cipsi_ijkl_batch* b = ctx->batch_cache;
+    const bool same = b && b->ngeom == n_geom && std::equal(n_ao, n_ao + n_geom, b->n.begin()) &&
+                      std::equal(n_mo, n_mo + n_geom, b->m.begin());
+    if (same) return cipsi_ijkl_batch_run(ctx, b, d_ao, d_c, ldc, d_mo, 1);
+    if (b) {
+        cipsi_ijkl_batch_destroy(ctx, b);
+        ctx->batch_cache = nullptr;
+    }
+    if ((rc = cipsi_ijkl_batch_create(ctx, n_geom, n_ao, n_mo, &b))) return rc;
+    ctx->batch_cache = b;
+    return cipsi_ijkl_batch_autotune(ctx, b, d_ao, d_c, ldc, d_mo);
 }
 
 int cipsi_ijkl_batch_num_classes(const cipsi_ijkl_batch* b) {

@@ -225,6 +225,10 @@ int cipsi_ijkl_release_workspace(cipsi_ijkl_ctx* ctx) {
     int rc = ctx_enter(ctx);
     if (rc) return rc;
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->batch_cache) {
+        cipsi_ijkl_batch_destroy(ctx, ctx->batch_cache);
+        ctx->batch_cache = nullptr;
+    }
     ctx_free(ctx->vsq);
     ctx_free(ctx->h);
     ctx_free(ctx->cr);

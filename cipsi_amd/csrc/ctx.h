@@ -9,6 +9,8 @@
 
 #include "common.h"
 
+struct cipsi_ijkl_batch;
+
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
@@ -43,6 +45,7 @@ struct cipsi_ijkl_ctx {
     DevBuf adj_small;  // energy/gradient: h_MO, hC, reduction partials
     DevBuf cons;     // integral consumers: bad-label count, energy
     void* blas = nullptr;  // rocblas_handle, created on first gradient call
+    cipsi_ijkl_batch* batch_cache = nullptr;  // plan of the last cipsi_ijkl_transform_batch_dev
     int ts_m = -1, ts_lt = -1;
     int64_t ts_total = 0;
     // tile plans (kernel variant index) for the two halves: defaults and per-size plans

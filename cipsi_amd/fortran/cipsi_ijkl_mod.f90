@@ -78,6 +78,17 @@ module cipsi_ijkl_mod
       real(c_double), intent(out) :: energy, grad(*)
     end function
 
+    ! A whole scan in one call: n_geom device pointers per argument (FP64 only; the
+    ! context keeps the plan of its last size list and replays its graph for the same list).
+    integer(c_int) function cipsi_ijkl_transform_batch_dev(ctx, n_geom, n_ao, n_mo, d_ao, d_c, &
+        ldc, d_mo, precision, n_slices) bind(C, name="cipsi_ijkl_transform_batch_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n_geom, precision, n_slices
+      integer(c_int), intent(in) :: n_ao(*), n_mo(*), ldc(*)
+      type(c_ptr), intent(in) :: d_ao(*), d_c(*), d_mo(*)
+    end function
+
     ! Closed-shell energy (no nuclear term), Fock (|F| < 1e-7 -> 0), J and K of
     ! fock.f:80-127 for noc doubly occupied MOs; all packed i >= j like h_mo_packed.
     integer(c_int) function cipsi_ijkl_fock(ctx, n_mo, noc, mo_packed, h_mo_packed, energy, &

@@ -47,6 +47,7 @@ EXPORTS = [
     "cipsi_ijkl_energy_grad", "cipsi_ijkl_energy_grad_dev",
     "cipsi_ijkl_batch_workspace_bytes", "cipsi_ijkl_batch_create", "cipsi_ijkl_batch_destroy",
     "cipsi_ijkl_batch_autotune", "cipsi_ijkl_batch_run", "cipsi_ijkl_batch_num_classes",
+    "cipsi_ijkl_transform_batch_dev",
     "cipsi_ijkl_batch_class_info", "cipsi_ijkl_batch_set_class_plan",
     "cipsi_ijkl_gather_dev", "cipsi_ijkl_fock", "cipsi_ijkl_fock_dev",
     "cipsi_ijkl_hdig_dev", "cipsi_ijkl_hntd_dev",
@@ -119,6 +120,8 @@ def load_library(path: str = LIB_PATH):
     L.cipsi_ijkl_batch_autotune.argtypes = [_vp, _vp, _vp, _vp, _ip, _vp]
     L.cipsi_ijkl_batch_run.argtypes = [_vp, _vp, _vp, _vp, _ip, _vp, _c_int]
     L.cipsi_ijkl_batch_num_classes.argtypes = [_vp]
+    L.cipsi_ijkl_transform_batch_dev.argtypes = [_vp, _c_int, _ip, _ip, _vp, _vp, _ip, _vp,
+                                                 _c_int, _c_int]
     L.cipsi_ijkl_batch_class_info.argtypes = [_vp, _c_int, _ip, _ip, _ip, _ip]
     L.cipsi_ijkl_batch_set_class_plan.argtypes = [_vp, _vp, _c_int, _c_int, _c_int]
     L.cipsi_ijkl_gather_dev.argtypes = [_vp, _c_int, _vp, _vp, _c_i64, _vp]
@@ -223,6 +226,26 @@ class Context:
             self.handle, n, m, _ptr(ao), _ptr(cm), ldc, _ptr(out), prec, n_slices,
             ctypes.byref(err)), "transform_packed")
         return (out, err.value) if return_err else out
+
+    def transform_batch(self, aos, cs, mos=None):
+        """A whole scan in one call (cipsi_ijkl_transform_batch_dev): lists of float64 CUDA
+        tensors, one packed AO block, C (N_AO x N_MO) and output per geometry.  The context
+        keeps the plan of its last size list: a new list builds and autotunes one on this
+        data, the same list again replays the captured graph on torch's current stream."""
+        import torch
+        ns = [int(c.shape[0]) for c in cs]
+        ms = [int(c.shape[1]) for c in cs]
+        if mos is None:
+            mos = [torch.empty(n_pairs(n_pairs(m)), dtype=torch.float64, device=a.device)
+                   for a, m in zip(aos, ms)]
+        ao_p, c_p, ldc, mo_p, keep = _batch_arrays(ns, ms, aos, cs, mos)
+        g = len(ns)
+        self._use_torch_stream()
+        self._check(self.lib.cipsi_ijkl_transform_batch_dev(
+            self.handle, g, (ctypes.c_int * g)(*ns), (ctypes.c_int * g)(*ms), ao_p, c_p, ldc,
+            mo_p, 0, 0), "transform_batch_dev")
+        self._batch_keep = keep    # column-major copies of C stay alive until the next call
+        return mos
 
     def itijkl(self, xx, ix, c, precision="fp64", n_slices=0, return_err=False):
         """pshf label stream (values, 32-bit labels) -> packed MO stream."""
@@ -490,31 +513,7 @@ class BatchPlan:
             pass
 
     def _arrays(self, aos, cs, mos):
-        g = len(self.n)
-        if not (len(aos) == len(cs) == len(mos) == g):
-            raise ValueError("one AO block, C and output per geometry")
-        ao_p = (_vp * g)()
-        c_p = (_vp * g)()
-        mo_p = (_vp * g)()
-        ldc = (ctypes.c_int * g)()
-        keep = []
-        for i in range(g):
-            n, m = self.n[i], self.m[i]
-            if tuple(cs[i].shape) != (n, m):
-                raise ValueError(f"geometry {i}: C must be {n} x {m}")
-            if aos[i].numel() != n_pairs(n_pairs(n)) or mos[i].numel() != n_pairs(n_pairs(m)):
-                raise ValueError(f"geometry {i}: AO block or output of the wrong length")
-            for t in (aos[i], mos[i]):
-                if not _is_torch(t) or not t.is_cuda or str(t.dtype) != "torch.float64" \
-                        or not t.is_contiguous():
-                    raise TypeError("batch inputs must be contiguous float64 CUDA tensors")
-            cm, ld = Context._cmat(cs[i])
-            keep.append(cm)
-            ao_p[i] = aos[i].data_ptr()
-            c_p[i] = cm.data_ptr()
-            mo_p[i] = mos[i].data_ptr()
-            ldc[i] = ld
-        return ao_p, c_p, ldc, mo_p, keep
+        return _batch_arrays(self.n, self.m, aos, cs, mos)
 
     def run(self, aos, cs, mos, use_graph=True):
         ao_p, c_p, ldc, mo_p, keep = self._arrays(aos, cs, mos)
@@ -546,6 +545,36 @@ class BatchPlan:
     def set_class_plan(self, k, var_half1, var_half2):
         self.ctx._check(self.lib.cipsi_ijkl_batch_set_class_plan(
             self.ctx.handle, self.handle, k, var_half1, var_half2), "batch_set_class_plan")
+
+
+def _batch_arrays(ns, ms, aos, cs, mos):
+    """Host arrays of device pointers for the batch entry points (and the column-major
+    copies of C that must stay alive while the launch runs)."""
+    g = len(ns)
+    if not (len(aos) == len(cs) == len(mos) == g):
+        raise ValueError("one AO block, C and output per geometry")
+    ao_p = (_vp * g)()
+    c_p = (_vp * g)()
+    mo_p = (_vp * g)()
+    ldc = (ctypes.c_int * g)()
+    keep = []
+    for i in range(g):
+        n, m = ns[i], ms[i]
+        if tuple(cs[i].shape) != (n, m):
+            raise ValueError(f"geometry {i}: C must be {n} x {m}")
+        if aos[i].numel() != n_pairs(n_pairs(n)) or mos[i].numel() != n_pairs(n_pairs(m)):
+            raise ValueError(f"geometry {i}: AO block or output of the wrong length")
+        for t in (aos[i], mos[i]):
+            if not _is_torch(t) or not t.is_cuda or str(t.dtype) != "torch.float64" \
+                    or not t.is_contiguous():
+                raise TypeError("batch inputs must be contiguous float64 CUDA tensors")
+        cm, ld = Context._cmat(cs[i])
+        keep.append(cm)
+        ao_p[i] = aos[i].data_ptr()
+        c_p[i] = cm.data_ptr()
+        mo_p[i] = mos[i].data_ptr()
+        ldc[i] = ld
+    return ao_p, c_p, ldc, mo_p, keep
 
 
 def batch_workspace_bytes(n_ao, n_mo):

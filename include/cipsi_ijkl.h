@@ -197,6 +197,18 @@ int cipsi_ijkl_batch_run(cipsi_ijkl_ctx *ctx, cipsi_ijkl_batch *plan, const doub
                          const double *const *d_c, const int *ldc, double *const *d_mo,
                          int use_graph);
 int cipsi_ijkl_batch_num_classes(const cipsi_ijkl_batch *plan);
+/* One call per scan, the batch entry point of the drop-in proposal (no counterpart in
+   the reference, whose scan starts one ijkl process per geometry): device pointers as in
+   cipsi_ijkl_batch_run, FP64 only (precision 0, n_slices 0; split-FP16 gives
+   CIPSI_ERR_UNSUPPORTED).  The context keeps one plan, for its last size list: a new
+   size list builds and autotunes a plan on this call's data (seconds for 64 geometries)
+   and synchronizes; the same size list again replays the captured graph, asynchronous
+   on the context stream.  cipsi_ijkl_release_workspace and cipsi_ijkl_destroy free the
+   plan. */
+int cipsi_ijkl_transform_batch_dev(cipsi_ijkl_ctx *ctx, int n_geom, const int *n_ao,
+                                   const int *n_mo, const double *const *d_ao,
+                                   const double *const *d_c, const int *ldc,
+                                   double *const *d_mo, int precision, int n_slices);
 int cipsi_ijkl_batch_class_info(const cipsi_ijkl_batch *plan, int k, int *row_blocks,
                                 int *n_geom, int *var_half1, int *var_half2);
 int cipsi_ijkl_batch_set_class_plan(cipsi_ijkl_ctx *ctx, cipsi_ijkl_batch *plan, int k,

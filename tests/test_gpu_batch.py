@@ -169,3 +169,25 @@ def test_scan_64_geometries(ctx):
     cf = synth.mo_reference_closed_form(Bs[g], c)
     assert np.max(np.abs(out[g].cpu().numpy() - cf)) <= 1e-11
     plan.close()
+
+
+def test_transform_batch_one_call(ctx):
+    """cipsi_ijkl_transform_batch_dev, the one-call form: the first call for a size list
+    builds and autotunes a plan on its data, a second call with the same sizes and new
+    data replays the captured graph, and a new size list replaces the plan.  Every
+    geometry equals the single-transform path bit for bit each time."""
+    print("[batch one-call]", flush=True)
+    for sizes, seed in ((MIXED[:5], 800), (MIXED[:5], 820), (MIXED[2:7], 840)):
+        aos, cs, _ = make_geoms(sizes, seed)
+        ref = single(ctx, aos, cs)
+        out = ctx.transform_batch(aos, cs)
+        torch.cuda.synchronize()
+        for g in range(len(sizes)):
+            assert torch.equal(out[g], ref[g]), f"seed {seed}, geometry {g}"
+    ctx.release_workspace()                      # frees the kept plan too
+    aos, cs, _ = make_geoms(MIXED[:2], 860)
+    ref = single(ctx, aos, cs)
+    out = ctx.transform_batch(aos, cs)
+    torch.cuda.synchronize()
+    for g in range(2):
+        assert torch.equal(out[g], ref[g])
