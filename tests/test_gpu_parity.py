@@ -291,3 +291,24 @@ def test_autotune_keeps_the_bits_and_built_variants(ctx, n, m):
         assert torch.equal(out, ref)
     finally:
         ctx.set_plan(n, m, *ctx.get_plan(1, 1))
+
+
+def test_empty_label_streams(ctx):
+    """A stream with no entries, or one whose first label is the terminator, unpacks to
+    an all-zero block (the block is zeroed first, itijkl.f:191-192, and the first zero
+    label ends the stream, itijkl.f:62-68), so every MO integral is zero: host and device
+    entry points."""
+    n, m = 7, 5
+    c = synth.make_case(n, m, 2 * n, 3.0, 51)["c"]
+    nout = synth.n_pairs(synth.n_pairs(m))
+    cases = [(np.zeros(0), np.zeros(0, dtype=np.int32)),
+             (np.array([0.0, 2.5, 1.0]),
+              np.array([0, formats.label_encode(1, 1, 1, 1), formats.label_encode(2, 1, 1, 1)],
+                       dtype=np.int32))]
+    for xx, ix in cases:
+        out = ctx.itijkl(xx, ix, c)
+        assert out.shape == (nout,) and not np.any(out)
+        if xx.size:
+            d = ctx.itijkl(dev(xx), torch.as_tensor(ix, device="cuda"), dev(c))
+            torch.cuda.synchronize()
+            assert d.numel() == nout and not torch.any(d)
