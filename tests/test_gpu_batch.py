@@ -191,3 +191,31 @@ def test_transform_batch_one_call(ctx):
     torch.cuda.synchronize()
     for g in range(2):
         assert torch.equal(out[g], ref[g])
+
+
+def test_batch_autotune_skips_variants_a_class_cannot_hold(ctx):
+    """A plan with a geometry of 13 row blocks (N_AO = 208): the batch autotuner never
+    picks the 12-wave variant for that class (it is built only up to 12 blocks), an
+    explicit class plan naming it is refused with CIPSI_ERR_UNSUPPORTED, and the tuned
+    outputs equal the single-transform path bit for bit."""
+    from cipsi_amd import ijkl
+    print("[batch 13 blocks]", flush=True)
+    sizes = [(208, 200), (40, 30)]
+    aos, cs, _ = make_geoms(sizes, 900)
+    ref = single(ctx, aos, cs)
+    plan = ijkl.BatchPlan(ctx, [n for n, _ in sizes], [m for _, m in sizes])
+    out = outputs(sizes)
+    shapes = ijkl.variants()
+    wide = [v for v, s in enumerate(shapes) if s[0] == 12]
+    classes = plan.autotune(aos, cs, out)
+    torch.cuda.synchronize()
+    for g in range(len(sizes)):
+        assert torch.equal(out[g], ref[g]), f"geometry {g}"
+    for k, (rb, ng, v1, v2) in enumerate(classes):
+        if rb > 12:
+            assert v1 not in wide and v2 not in wide
+            for v in wide:
+                with pytest.raises(ijkl.IjklError) as e:
+                    plan.set_class_plan(k, v, v)
+                assert e.value.status == -7
+    plan.close()
