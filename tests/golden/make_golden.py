@@ -1,7 +1,7 @@
 """Generate the golden fixtures of the ijkl transform from the reference program itself.
 
-Runs in the survey/build container only (it needs the reference sources under
-/root/reference to build oracle/_ref/ijkl_ref with `make -C oracle`).  For each case it
+Runs only on a machine that has the reference sources under /root/reference (it
+builds oracle/_ref/ijkl_ref from them with `make -C oracle`).  For each case it
 writes the reference's own input files (info, pqrs, namelists on stdin), runs the
 flang-built reference ijkl, parses f50 / f25 / ijom and stores the inputs that are
 not regenerable bit-for-bit (C) plus the outputs as .npz fixtures next to this script.
