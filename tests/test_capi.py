@@ -47,3 +47,28 @@ def test_create_without_gpu_reports_nodevice():
     with pytest.raises(ijkl.IjklError) as e:
         ijkl.Context(0)
     assert e.value.status == -8
+
+
+@pytest.mark.skipif(not os.path.exists(ijkl.LIB_PATH), reason="library not built")
+def test_every_context_entry_point_rejects_a_null_context():
+    """Each function of the header whose first parameter is the context returns
+    CIPSI_ERR_ARG for a NULL context (pointers NULL, numbers 0) before touching a device,
+    so a caller's mistake is a status, never a crash."""
+    import ctypes
+    txt = open(os.path.join(ROOT, "include", "cipsi_ijkl.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = sorted(set(re.findall(r"\b(cipsi_ijkl_\w+)\s*\(\s*cipsi_ijkl_ctx\s*\*", txt)))
+    assert len(names) > 20
+    lib = ijkl.load_library()
+    for name in names:
+        fn = getattr(lib, name)
+        args = []
+        for t in fn.argtypes:
+            if t in (ctypes.c_double, ctypes.c_float):
+                args.append(0.0)
+            elif t in (ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_long,
+                       ctypes.c_longlong, ctypes.c_uint, ctypes.c_size_t):
+                args.append(0)
+            else:
+                args.append(None)
+        assert fn(*args) == -1, name
