@@ -325,11 +325,12 @@ int cipsi_ijkl_energy_grad(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double
     double* d_grad = (double*)ctx->io_out.ptr;
     double* d_e = (double*)((char*)ctx->io_out.ptr + s_grad);
     hipStream_t st = ctx->stream;
-    if (hipMemcpyAsync(ctx->io_in.ptr, ao, s_ao, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(d_h, h_packed, s_h, hipMemcpyHostToDevice, st) != hipSuccess ||
+    // the two large arrays (AO block, pair density) go through the pinned bounce slots
+    if ((rc = ctx_copy_h2d(ctx, ctx->io_in.ptr, ao, s_ao))) return rc;
+    if ((rc = ctx_copy_h2d(ctx, d_Gam, Gamma_packed, s_G))) return rc;
+    if (hipMemcpyAsync(d_h, h_packed, s_h, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(d_c, c, s_c, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(d_gam, gamma, s_g, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(d_Gam, Gamma_packed, s_G, hipMemcpyHostToDevice, st) != hipSuccess)
+        hipMemcpyAsync(d_gam, gamma, s_g, hipMemcpyHostToDevice, st) != hipSuccess)
         return CIPSI_ERR_HIP;
     if ((rc = energy_impl(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr, d_h, d_c, ldc, d_gam,
                           d_Gam, d_e, grad ? d_grad : nullptr)))

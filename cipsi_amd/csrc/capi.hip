@@ -441,7 +441,9 @@ static int pin_wait(cipsi_ijkl_ctx* ctx, int s) {
     return CIPSI_OK;
 }
 
-static int copy_h2d(cipsi_ijkl_ctx* ctx, void* d, const void* h, size_t bytes) {
+}  // extern "C"
+
+int ctx_copy_h2d(cipsi_ijkl_ctx* ctx, void* d, const void* h, size_t bytes) {
     int rc = pin_ready(ctx);
     if (rc) return rc;
     const size_t chunk = cipsi_ijkl_ctx::kPinBytes;
@@ -457,6 +459,8 @@ static int copy_h2d(cipsi_ijkl_ctx* ctx, void* d, const void* h, size_t bytes) {
     }
     return CIPSI_OK;
 }
+
+extern "C" {
 
 // Ordered after the work already queued on the context stream; returns when the caller's
 // buffer holds the data.
@@ -554,7 +558,7 @@ int cipsi_ijkl_transform_packed(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
     if ((rc = ctx_alloc(ctx, ctx->io_out, sizeof(double) * nout))) return rc;
     double* d_c;
     if ((rc = upload_c(ctx, n_ao, n_mo, c, ldc, &d_c))) return rc;
-    if ((rc = copy_h2d(ctx, ctx->io_in.ptr, ao, sizeof(double) * nin))) return rc;
+    if ((rc = ctx_copy_h2d(ctx, ctx->io_in.ptr, ao, sizeof(double) * nin))) return rc;
     double vmax = 0.0;
     if ((rc = scan_uploaded(ctx, (const double*)ctx->io_in.ptr, nin, nullptr, &vmax))) return rc;
     if ((rc = cipsi_ijkl_transform_packed_dev(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr, d_c,
@@ -583,8 +587,8 @@ int cipsi_ijkl_transform_labels(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
     double* d_c;
     if ((rc = upload_c(ctx, n_ao, n_mo, c, ldc, &d_c))) return rc;
     if (n_entries > 0) {
-        if ((rc = copy_h2d(ctx, ctx->io_in.ptr, xx, sizeof(double) * n_entries))) return rc;
-        if ((rc = copy_h2d(ctx, ctx->io_in2.ptr, labels, sizeof(int32_t) * n_entries))) return rc;
+        if ((rc = ctx_copy_h2d(ctx, ctx->io_in.ptr, xx, sizeof(double) * n_entries))) return rc;
+        if ((rc = ctx_copy_h2d(ctx, ctx->io_in2.ptr, labels, sizeof(int32_t) * n_entries))) return rc;
     }
     double vmax = 0.0;          // 8 max |xx| before the first zero label (transd's largest scale)
     if ((rc = scan_uploaded(ctx, (const double*)ctx->io_in.ptr, n_entries,

@@ -84,6 +84,9 @@ struct StageTimer {
 };
 
 int ctx_alloc(cipsi_ijkl_ctx* ctx, DevBuf& b, size_t bytes);
+// Host -> device copy through the context's pinned bounce slots (host threads fill a
+// slot while the DMA engine moves the previous one); ordered on the context stream.
+int ctx_copy_h2d(cipsi_ijkl_ctx* ctx, void* d, const void* h, size_t bytes);
 int ctx_enter(cipsi_ijkl_ctx* ctx);
 int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int64_t* total);
 int run_pad_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_c, int ldc, double** cr);
