@@ -40,10 +40,21 @@ static int map_hip(cipsi_ijkl_ctx* ctx, hipError_t e, const char* where) {
         if (_e != hipSuccess) return map_hip((ctx), _e, #expr);     \
     } while (0)
 
+// Grow-only workspace.  A buffer that is too small is retired, not freed: work already
+// queued, or a graph captured from an earlier call, may still name it.  Retired buffers
+// are freed by release_workspace / destroy.
 int ctx_alloc(cipsi_ijkl_ctx* ctx, DevBuf& b, size_t bytes) {
     if (b.bytes >= bytes && b.ptr) return CIPSI_OK;
-    if (b.ptr) { (void)hipFree(b.ptr); b.ptr = nullptr; b.bytes = 0; }
     if (bytes == 0) return CIPSI_OK;
+    if (b.ptr) {
+        try {
+            ctx->retired.push_back(b);
+        } catch (...) {
+            return CIPSI_ERR_NOMEM;
+        }
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
     hipError_t e = hipMalloc(&b.ptr, bytes);
     if (e != hipSuccess) { b.ptr = nullptr; return map_hip(ctx, e, "hipMalloc"); }
     b.bytes = bytes;
@@ -71,24 +82,35 @@ static int check_dims(int n_ao, int n_mo, int ldc) {
 
 static inline int round16(int x) { return (x + 15) & ~15; }
 
-// Half-2 work table: tile_start[l] = sum_{l'<l} (l'+1) * ceil((l'+1)/lt).
+// Half-2 work table: tile_start[l] = sum_{l'<l} (l'+1) * ceil((l'+1)/lt).  One table per
+// (m, lt), built on first use (a synchronous upload, so not under graph capture) and never
+// rewritten, so a captured graph keeps a valid table whatever later calls use.
 int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int64_t* total) {
-    if (ctx->ts_m != m || ctx->ts_lt != lt) {
-        // the table may still be read by kernels queued on the context stream
-        if (ctx->ts_m != -1) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const auto key = std::make_pair(m, lt);
+    auto it = ctx->tile_tables.find(key);
+    if (it == ctx->tile_tables.end()) {
         std::vector<int64_t> t(m + 1);
         t[0] = 0;
         for (int l = 0; l < m; ++l) t[l + 1] = t[l] + (int64_t)(l + 1) * ((l + lt) / lt);
-        int rc = ctx_alloc(ctx, ctx->tile_start, sizeof(int64_t) * (m + 1));
+        cipsi_ijkl_ctx::TileTable tab;
+        int rc = ctx_alloc(ctx, tab.buf, sizeof(int64_t) * (m + 1));
         if (rc) return rc;
-        HIP_TRY(ctx, hipMemcpy(ctx->tile_start.ptr, t.data(), sizeof(int64_t) * (m + 1),
-                               hipMemcpyHostToDevice));
-        ctx->ts_m = m;
-        ctx->ts_lt = lt;
-        ctx->ts_total = t[m];
+        hipError_t e = hipMemcpy(tab.buf.ptr, t.data(), sizeof(int64_t) * (m + 1),
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(tab.buf.ptr);
+            return map_hip(ctx, e, "hipMemcpy(tile table)");
+        }
+        tab.total = t[m];
+        try {
+            it = ctx->tile_tables.emplace(key, tab).first;
+        } catch (...) {
+            (void)hipFree(tab.buf.ptr);
+            return CIPSI_ERR_NOMEM;
+        }
     }
-    *out = (const int64_t*)ctx->tile_start.ptr;
-    *total = ctx->ts_total;
+    *out = (const int64_t*)it->second.buf.ptr;
+    *total = it->second.total;
     return CIPSI_OK;
 }
 
@@ -240,13 +262,15 @@ int cipsi_ijkl_release_workspace(cipsi_ijkl_ctx* ctx) {
     ctx_free(ctx->io_c);
     ctx_free(ctx->scal);
     ctx_free(ctx->scan);
-    ctx_free(ctx->tile_start);
     ctx_free(ctx->slices);
     ctx_free(ctx->adj_g);
     ctx_free(ctx->adj_gh);
     ctx_free(ctx->adj_small);
     ctx_free(ctx->cons);
-    ctx->ts_m = -1;
+    for (auto& kv : ctx->tile_tables) ctx_free(kv.second.buf);
+    ctx->tile_tables.clear();
+    for (auto& b : ctx->retired) ctx_free(b);
+    ctx->retired.clear();
     return CIPSI_OK;
 }
 
@@ -330,7 +354,7 @@ int cipsi_ijkl_unpack_labels_dev(cipsi_ijkl_ctx* ctx, int n_ao, const double* d_
     const int64_t nblk = npairs(npairs(n_ao));
     if ((rc = ctx_alloc(ctx, ctx->owner, sizeof(uint32_t) * nblk))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->scal, 64))) return rc;
-    unsigned long long* sc = (unsigned long long*)ctx->scal.ptr;   // 3 scalars
+    unsigned long long* sc = (unsigned long long*)ctx->scal.ptr;   // 4 scalars
     int* err_flag = (int*)((char*)ctx->scal.ptr + 32);
     {
         StageTimer tm(ctx, "unpack");

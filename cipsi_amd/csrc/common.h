@@ -5,7 +5,7 @@
 //   P_A = N(N+1)/2 AO pairs, P_M = M(M+1)/2 MO pairs
 //   packed AO block   V[pair0(ij, kl)], ij >= kl            (transd.f:369-371)
 //   half-transformed  H[pq * P_M + pair0(l, k)], l >= k      (tpqkld.f:91-102)
-//   packed MO stream  out[pair0(kl, pair0(j, i))], ij <= kl   (tijkld.f:159-186)
+//   packed MO stream  out[pair0(kl, pair0(j, i))], ij <= kl   (tijkld.f:31-79)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

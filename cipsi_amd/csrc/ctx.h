@@ -38,7 +38,13 @@ struct cipsi_ijkl_ctx {
     hipEvent_t pin_ev[kPinSlots] = {};
     bool pin_busy[kPinSlots] = {};
     DevBuf io_in, io_in2, io_out, io_c;   // staging for host-pointer entry points
-    DevBuf tile_start;                    // half-2 work table
+    // Half-2 work tables, one per (N_MO, columns per work-group), written once and kept
+    // until release_workspace: a graph captured from an earlier call keeps its pointer.
+    struct TileTable { DevBuf buf; int64_t total = 0; };
+    std::map<std::pair<int, int>, TileTable> tile_tables;
+    // Buffers replaced by a larger allocation.  They are not freed before
+    // release_workspace / destroy, so a captured graph that still names them stays valid.
+    std::vector<DevBuf> retired;
     DevBuf slices;   // split-FP16 operand slices
     DevBuf adj_g;    // energy/gradient: packed MO integrals
     DevBuf adj_gh;   // energy/gradient: scaled pair density Gh[b][a] = m(b) Gamma(a,b)
@@ -46,8 +52,6 @@ struct cipsi_ijkl_ctx {
     DevBuf cons;     // integral consumers: bad-label count, energy
     void* blas = nullptr;  // rocblas_handle, created on first gradient call
     cipsi_ijkl_batch* batch_cache = nullptr;  // plan of the last cipsi_ijkl_transform_batch_dev
-    int ts_m = -1, ts_lt = -1;
-    int64_t ts_total = 0;
     // tile plans (kernel variant index) for the two halves: defaults and per-size plans
     int var_half1 = 2;
     int var_half2 = 2;

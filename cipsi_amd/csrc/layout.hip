@@ -37,23 +37,41 @@ static inline unsigned grid_for(int64_t n, int threads, unsigned cap = 65536u) {
 //   fix     only when a valid label lies after the first zero (the stale tail of the
 //           last record): the owners are cleared and re-claimed up to the stream end;
 //   resolve one thread per position: the owner's value times 2^(doubling), or 0.
+//   noncanon only when a valid label is not in pshf's canonical order (i >= j, k >= l,
+//           ij >= kl): resolve derives the doubling from the position, which for such a
+//           label differs from transd's, which tests the label's own fields
+//           (transd.f:211-218); the owners among them are rewritten with transd's value.
 // Scalars sc[]: 0 = stream end (first zero label), 1 = first bad label, 2 = last valid
-// label + 1 (0 = none), all as unsigned 64-bit for the atomics.
+// label + 1 (0 = none), 3 = nonzero if any valid label is non-canonical, all as unsigned
+// 64-bit for the atomics.
 
 __global__ void k_init_label_scalars(unsigned long long* sc, int64_t n, int* err_flag) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         sc[0] = (unsigned long long)n;
         sc[1] = (unsigned long long)n;
         sc[2] = 0ull;
+        sc[3] = 0ull;
         *err_flag = 0;
     }
 }
 
+// Fields of a label (transd.f:94-97), 1-based.
+__device__ inline void label_fields(int32_t lab, int& i, int& j, int& k, int& l) {
+    const uint32_t u = (uint32_t)lab;
+    i = (int)((u >> 24) & 255u);
+    j = (int)((u >> 16) & 255u);
+    k = (int)((u >> 8) & 255u);
+    l = (int)(u & 255u);
+}
+
+__device__ inline bool label_canonical(int i, int j, int k, int l) {
+    return i >= j && k >= l && i * (i - 1) / 2 + j >= k * (k - 1) / 2 + l;
+}
+
 // 1-based fields (transd.f:91-97) -> 0-based packed position; false if out of range.
 __device__ inline bool decode_label(int32_t lab, int n_ao, int64_t* pos) {
-    const uint32_t u = (uint32_t)lab;
-    const int i = (int)((u >> 24) & 255u), j = (int)((u >> 16) & 255u);
-    const int k = (int)((u >> 8) & 255u), l = (int)(u & 255u);
+    int i, j, k, l;
+    label_fields(lab, i, j, k, l);
     if (i < 1 || j < 1 || k < 1 || l < 1 || i > n_ao || j > n_ao || k > n_ao || l > n_ao)
         return false;
     const int64_t ij1 = (int64_t)i * (i - 1) / 2 + j;
@@ -98,6 +116,7 @@ __global__ __launch_bounds__(kLabelThreads) void k_claim_scan(const int32_t* __r
                                                                uint32_t* __restrict__ owner,
                                                                unsigned long long* sc) {
     unsigned long long zmin = ~0ull, bmin = ~0ull, vmax = 0ull;
+    bool nonc = false;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int32_t lab = labels[i];
@@ -109,9 +128,36 @@ __global__ __launch_bounds__(kLabelThreads) void k_claim_scan(const int32_t* __r
         } else {
             atomicMax(owner + pos, (uint32_t)(i + 1));
             vmax = max(vmax, (unsigned long long)(i + 1));
+            int a, b, c, d;
+            label_fields(lab, a, b, c, d);
+            nonc = nonc || !label_canonical(a, b, c, d);
         }
     }
+    if (__ballot(nonc) != 0ull && (threadIdx.x & 63) == 0) atomicOr(&sc[3], 1ull);
     block_commit(zmin, bmin, vmax, sc);
+}
+
+// Non-canonical owners: the value transd stores, xx doubled for i == j, k == l and
+// (i, j) == (k, l) of the label itself (transd.f:210-218), over the resolve's value.
+__global__ void k_noncanon_fix(const double* __restrict__ xx, const int32_t* __restrict__ labels,
+                               int n_ao, const uint32_t* __restrict__ owner,
+                               const unsigned long long* sc, double* __restrict__ out) {
+    if (sc[3] == 0ull) return;
+    const int64_t n = (int64_t)sc[0];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+        const int32_t lab = labels[t];
+        int64_t pos;
+        if (!decode_label(lab, n_ao, &pos)) continue;
+        int i, j, k, l;
+        label_fields(lab, i, j, k, l);
+        if (label_canonical(i, j, k, l) || owner[pos] != (uint32_t)(t + 1)) continue;
+        double x = xx[t];
+        if (i == j) x = x + x;
+        if (k == l) x = x + x;
+        if (i == k && j == l) x = x + x;
+        out[pos] = x;
+    }
 }
 
 // A valid label after the stream end claimed a position: clear and re-claim up to the end.
@@ -236,6 +282,9 @@ hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t
     else
         hipLaunchKernelGGL(k_resolve<false>, rg, dim3(kLabelThreads), 0, st, xx, nblk, owner, sc,
                            err_flag, ao_packed);
+    if (n_entries > 0)
+        hipLaunchKernelGGL(k_noncanon_fix, dim3(grid_for(n_entries, 256, 2048)), dim3(256), 0, st,
+                           xx, labels, n_ao, owner, sc, ao_packed);
     return hipGetLastError();
 }
 

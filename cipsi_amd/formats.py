@@ -205,7 +205,7 @@ def read_f50(path, n_mo: int, n_vec_mono: int | None = None):
 def write_f50(path, mo_packed: np.ndarray, hmo_packed: np.ndarray) -> int:
     """The f50 stream as tijkld/wtrbld and tmono write it: full 8192-double records; the
     last MO record is written full length, its tail holding what the previous record
-    left in the buffer (tijkld.f:176-182,192-195); then one record with h_MO."""
+    left in the buffer (tijkld.f:68-74,84-88); then one record with h_MO."""
     buf = np.zeros(LSIZE)
     n = mo_packed.size
     nrec = 0

@@ -39,15 +39,21 @@ def pair_index_arrays(n: int):
     return a.astype(np.int64), b.astype(np.int64)
 
 
-def eri_factors(n_ao: int, rank: int, decay: float, seed: int) -> np.ndarray:
-    """B[P, pair0(i,j)] for P < rank: z * exp(-|i-j|/decay) rounded to the 2^-17 grid."""
-    if rank > 4096:
+def eri_factors(n_ao: int, rank: int, decay: float, seed: int, grid: bool = True) -> np.ndarray:
+    """B[P, pair0(i,j)] for P < rank: z * exp(-|i-j|/decay) rounded to the 2^-17 grid.
+
+    grid=False keeps the full 53-bit mantissas (the same z and envelope, unrounded): the
+    integrals are then ordinary floating-point sums, not exact, and they carry low-order
+    bits below any fixed grid, as integrals from a real integral code do."""
+    if grid and rank > 4096:
         raise ValueError("rank > 4096 would break the exactness of the integral sums")
     rng = np.random.default_rng(seed)
     a, b = pair_index_arrays(n_ao)
     env = np.exp(-np.abs(a - b) / float(decay))
     z = rng.standard_normal((rank, a.size))
     z = np.clip(z, -7.99, 7.99)
+    if not grid:
+        return z * env
     B = np.rint(z * env * 2.0**B_GRID_BITS) * 2.0**-B_GRID_BITS
     return B
 
@@ -67,7 +73,8 @@ def eri_packed(B: np.ndarray, row_block: int = 2048) -> np.ndarray:
 
 
 def eri_packed_torch(B: np.ndarray, device, row_block: int = 2048):
-    """Same as eri_packed, computed with torch on `device` (exact, so identical bits)."""
+    """Same as eri_packed, computed with torch on `device` (exact, so identical bits, for
+    grid factors; for full-mantissa factors an ordinary rounded product)."""
     import torch
     Bd = torch.as_tensor(B, dtype=torch.float64, device=device)
     P = B.shape[1]

@@ -14,12 +14,17 @@
  *              columns /om/ c (vector.f:38-41, after rotat2.f:50-60 frozen compaction).
  *   h_half     P_A x P_M doubles, h_half[pq*P_M + pair0(l,k)] (tpqkld.f:91-102).
  *   mo_packed  P_M(P_M+1)/2 doubles, mo_packed[pair0(kl, pair0(j,i))] = (ij|kl), i<=j,
- *              ij<=kl, k<=l: the order tijkld streams to f50 (tijkld.f:159-186).
+ *              ij<=kl, k<=l: the order tijkld streams to f50 (tijkld.f:31-79).
  *
  * Every entry point returns CIPSI_OK (0) or a negative status; none calls exit().
  * Host-pointer entry points are synchronous and retain no pointer.  *_dev entry
  * points take device pointers, are asynchronous on the context stream and are
- * completed by cipsi_ijkl_synchronize().  A context is not thread-safe; use one
+ * completed by cipsi_ijkl_synchronize() (except the label-stream ones, which read a
+ * bad-label flag back and synchronize before returning).  After one warm-up call of
+ * the same size, precision and plan, the *_dev transforms of a packed block launch only
+ * kernels and may be captured in a graph; the graph stays valid until
+ * cipsi_ijkl_release_workspace() or cipsi_ijkl_destroy() (a workspace that must grow
+ * retires the old buffers instead of freeing them).  A context is not thread-safe; use one
  * context per host thread (the reference runs one process per geometry,
  * Autocip_13/autocip13.f90:525-567).
  */
@@ -80,7 +85,13 @@ int cipsi_ijkl_transform_packed(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo,
 /* Same from the pshf/cval label stream (itijkl.f:57-68 + transd.f): xx and labels
    hold n_entries values and 32-bit labels i<<24|j<<16|k<<8|l (iword_func.f:2-52,
    1-based indices, values pre-divided as pshf/qout.f:73-77 does); the first zero
-   label ends the stream; a later duplicate overwrites an earlier one. */
+   label ends the stream; a later duplicate overwrites an earlier one.  Labels are
+   expected in pshf's canonical order i>=j, k>=l, ij>=kl (pshf/qout.f:52-69).  Like the
+   reference's C1 route, which stores the label as read (transd.f:356,369-372: the
+   canonicalisation there applies to symmetry images only), a non-canonical label is
+   stored at pair0(pair0(i,j), pair0(k,l)) as given, doubled for i==j, k==l and
+   (i,j)==(k,l) of its own fields (transd.f:211-218); CIPSI_ERR_LABEL is returned only
+   for a field outside 1..n_ao. */
 int cipsi_ijkl_transform_labels(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, const double *xx,
                                 const int32_t *labels, int64_t n_entries, const double *c,
                                 int ldc, double *mo_packed, int precision, int n_slices,

@@ -27,7 +27,7 @@
  *   AO packed supermatrix  V[pair0(ij, kl)], ij >= kl    (transd.f:369-371)
  *   C  column-major N x M, C(mu, m) = c[mu + m*ldc]       (vector.f:38-41)
  *   half-transformed  H[npq * P_M + ilk], ilk = pair0(l, k), l >= k  (tpqkld.f:91-102)
- *   MO packed  out[pair0(ilk, nij)], nij = pair0(j, i), nij <= ilk   (tijkld.f:159-186)
+ *   MO packed  out[pair0(ilk, nij)], nij = pair0(j, i), nij <= ilk   (tijkld.f:31-79)
  *
  * The *_range variants compute a contiguous sub-range of the outer loop with
  * exactly the same per-element arithmetic, so a sampled or OpenMP-parallel run

@@ -3,6 +3,8 @@
 Bar: FP64 path within 1e-12 absolute of the oracle (which is bit-exact with the
 reference program) on the same inputs; layout/ordering checks are exact.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -103,6 +105,16 @@ def test_cfg1_n120_frozen10_against_reference(ctx):
     # whole stream against the independent closed form (different rounding path)
     cf = synth.mo_reference_closed_form(case["B"], case["c"])
     assert np.max(np.abs(o - cf)) <= 1e-11
+    # whole stream against the oracle's whole transform (every AO pair through tpqkld,
+    # every MO pair of the oracle's H through tijkld) at the 1e-12 bar
+    nth = max(1, min(16, os.cpu_count() or 1))
+    pa, pm = 120 * 121 // 2, 110 * 111 // 2
+    h_ref = oracle.tpqkld_rows(case["ao_packed"], case["c"], np.arange(pa, dtype=np.int64),
+                               nthreads=nth)
+    o_ref = oracle.tijkld_rows(h_ref, case["c"], np.arange(pm, dtype=np.int64), nthreads=nth)
+    d = float(np.max(np.abs(o - o_ref)))
+    print(f"[cfg1] max|gpu - oracle| over all {o.size} integrals = {d:.3e}", flush=True)
+    assert d <= TOL
 
 
 def test_errors_reported_not_fatal(ctx):
@@ -200,6 +212,26 @@ def test_label_stream_duplicates_and_stale_tail(ctx):
     np.testing.assert_array_equal(got.cpu().numpy(), want)
 
 
+def test_non_canonical_labels_stored_as_the_reference_does(ctx):
+    """The reference's C1 route stores a label as read (transd.f:356,369-372): a label
+    with i < j, k < l or ij < kl lands at pair0(pair0(i,j), pair0(k,l)) as given (with
+    every field in 1..N_AO that position is always inside the block).  The device unpack
+    equals the oracle's transd (the restatement pinned to the flang-built reference)
+    bit for bit on such labels."""
+    n = 7
+    print("[labels-noncanonical]", flush=True)
+    rng = np.random.default_rng(420)
+    lab = [(2, 5, 1, 1), (4, 4, 3, 6), (1, 2, 7, 7), (3, 1, 2, 2), (6, 6, 6, 5), (1, 7, 2, 7)]
+    xx = np.concatenate([rng.standard_normal(len(lab)), [0.0]])
+    ix = np.array([formats.label_encode(*t) for t in lab] + [0], dtype=np.int32)
+    want = oracle.transd(n, xx, ix)
+    got = ctx.transd(dev(xx), dev(ix), n)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(got.cpu().numpy(), want)
+    # (1,7|2,7) lands where (1,2|7,7) did and, being later, overwrites it
+    assert np.count_nonzero(want) == len(lab) - 1
+
+
 def test_transform_under_graph_capture(ctx):
     """A caller can capture the device transform in a CUDA graph (the library launches
     only kernels on the caller's stream once its workspace exists); each replay, with new
@@ -230,6 +262,48 @@ def test_transform_under_graph_capture(ctx):
             g.replay()
             torch.cuda.synchronize()
             assert torch.equal(out, refs[k]), f"replay {rep} case {k}"
+
+
+def test_captured_graph_survives_later_calls(ctx):
+    """A captured transform keeps working after later eager calls on the same context
+    that regrow its workspace (a larger N_AO), need other half-2 work tables (another
+    N_MO, the split path's tile width) or change the size's tile plan: grown buffers are
+    retired, not freed, and work tables are never rewritten, until release_workspace."""
+    n, m = 40, 24
+    print(f"[graph-later] n={n} m={m}", flush=True)
+    case = synth.make_case(n, m, 2 * n, 6.0, 710)
+    V, c = dev(case["ao_packed"]), dev(case["c"])
+    ref = ctx.transform(V, c)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        ctx.transform(V, c)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = ctx.transform(V, c)
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    plan = ctx.get_plan(n, m)
+    try:
+        big = synth.make_case(96, 80, 192, 6.0, 711)        # regrows vsq, h, cr
+        ctx.transform(dev(big["ao_packed"]), dev(big["c"]))
+        other_m = np.ascontiguousarray(case["c"][:, :30])   # a new (N_MO, width) table
+        ctx.transform(V, dev(other_m))
+        ctx.transform(V, c, precision="split_fp16")          # the split path's table
+        ctx.set_plan(n, m, 0, 4)                             # a new plan for this size
+        ctx.transform(V, c)
+        torch.cuda.synchronize()
+        for rep in range(2):
+            out.zero_()
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref), f"replay {rep} after later calls"
+    finally:
+        ctx.set_plan(n, m, *plan)
 
 
 def test_host_entry_input_scan(ctx):

@@ -91,6 +91,33 @@ def test_split_n240_error_and_energy(ctx):
     assert abs(e_split - e_ref) <= TOL
 
 
+@pytest.mark.parametrize("n,m,rank,decay,seed", [
+    (40, 40, 120, 8.0, 2024), (121, 111, 363, 15.0, 2025), (240, 240, 720, 20.0, 2026)])
+def test_split_full_mantissa_integrals(ctx, n, m, rank, decay, seed):
+    """AO integrals with full 53-bit mantissas (unrounded ERI factors).  The grid
+    integrals of the other tests are multiples of 2^-34 below 2^13, which six base-256
+    digits of each AO row hold exactly; these carry bits below every fixed grid, so the
+    digit truncation of the AO operand (half 1, step 1) is exercised.  Still <= 1e-10
+    against the FP64 path, and the FP64 path still agrees with the oracle on a sample."""
+    import oracle
+    print(f"[split full-mantissa] n={n} m={m}", flush=True)
+    B = synth.eri_factors(n, rank, decay, seed, grid=False)
+    ao = synth.eri_packed_torch(B, "cuda")
+    scaled = ao * 2.0**34
+    assert bool((scaled != torch.round(scaled)).any()), "integrals are on the 2^-34 grid"
+    del scaled
+    c = torch.as_tensor(np.ascontiguousarray(synth.orthonormal(n, seed + 1)[:, :m]), device="cuda")
+    ref = ctx.transform(ao, c)
+    out = ctx.transform(ao, c, precision="split_fp16")
+    err = float((out - ref).abs().max())
+    print(f"[split full-mantissa] n={n} m={m} maxabs={err:.3e} "
+          f"vmax={float(ref.abs().max()):.3e}", flush=True)
+    assert err <= TOL
+    if n <= 40:
+        o_ref = oracle.ijkl(ao.cpu().numpy(), c.cpu().numpy())
+        assert float(np.max(np.abs(ref.cpu().numpy() - o_ref))) <= 1e-12
+
+
 def test_split_host_entry_error_estimate(ctx):
     ao, c = inputs(30, 26, 60, 5.0, 11)
     ao_h, c_h = ao.cpu().numpy(), c.cpu().numpy()
