@@ -231,6 +231,11 @@ int cipsi_ijkl_create(cipsi_ijkl_ctx** out, int device) {
         return CIPSI_ERR_HIP;
     }
     ctx->stream = ctx->own_stream;
+    if (int rc = noalloc_setup(device)) {       // the no-allocation entry point's workspace
+        (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return rc;
+    }
     if (const char* v = std::getenv("CIPSI_IJKL_VARIANTS")) {     // "h1,h2" kernel variants
         int v1 = -1, v2 = -1;
         if (std::sscanf(v, "%d,%d", &v1, &v2) == 2 && v1 >= 0 && v1 < kNumHalfVariants && v2 >= 0 &&

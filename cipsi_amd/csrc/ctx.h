@@ -100,6 +100,10 @@ int run_half_variant(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* 
                      const double* cr, double* dst, int precision, int n_slices, int var);
 int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half, int n, int m, int precision);
 
+// No-allocation transform (noalloc.hip): resolves the device address of the static
+// workspace on `device` (current) once per process; called by cipsi_ijkl_create.
+int noalloc_setup(int device);
+
 // energy / gradient (adjoint.hip)
 void ctx_rocblas_destroy(cipsi_ijkl_ctx* ctx);
 

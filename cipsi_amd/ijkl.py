@@ -40,7 +40,7 @@ EXPORTS = [
     "cipsi_ijkl_set_stream", "cipsi_ijkl_reset_stream", "cipsi_ijkl_synchronize", "cipsi_ijkl_workspace_bytes",
     "cipsi_ijkl_release_workspace", "cipsi_ijkl_transform_packed", "cipsi_ijkl_transform_labels",
     "cipsi_ijkl_transform_packed_dev", "cipsi_ijkl_transform_labels_dev",
-    "cipsi_ijkl_unpack_labels_dev", "cipsi_ijkl_half1_dev", "cipsi_ijkl_half2_dev",
+    "cipsi_ijkl_transform_packed_noalloc_dev", "cipsi_ijkl_unpack_labels_dev", "cipsi_ijkl_half1_dev", "cipsi_ijkl_half2_dev",
     "cipsi_ijkl_tmono", "cipsi_ijkl_enable_timing", "cipsi_ijkl_read_timing",
     "cipsi_ijkl_reset_timing", "cipsi_ijkl_autotune", "cipsi_ijkl_set_plan", "cipsi_ijkl_get_plan",
     "cipsi_ijkl_num_variants", "cipsi_ijkl_variant_info",
@@ -96,6 +96,8 @@ def load_library(path: str = LIB_PATH):
                                                   _c_int, _c_int]
     L.cipsi_ijkl_transform_labels_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp,
                                                   _c_int, _vp, _c_int, _c_int]
+    L.cipsi_ijkl_transform_packed_noalloc_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_int,
+                                                          _vp]
     L.cipsi_ijkl_unpack_labels_dev.argtypes = [_vp, _c_int, _vp, _vp, _c_i64, _vp]
     L.cipsi_ijkl_half1_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _c_int]
     L.cipsi_ijkl_half2_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _c_int]
@@ -226,6 +228,26 @@ class Context:
             self.handle, n, m, _ptr(ao), _ptr(cm), ldc, _ptr(out), prec, n_slices,
             ctypes.byref(err)), "transform_packed")
         return (out, err.value) if return_err else out
+
+    def transform_noalloc(self, ao_packed, c, out=None):
+        """The FP64 transform through cipsi_ijkl_transform_packed_noalloc_dev: float64 CUDA
+        tensors; the library allocates nothing (its workspace is a static device array, so
+        N_AO <= 48).  `out` (optional) receives the packed MO stream."""
+        import torch
+        cm, ldc = self._cmat(c)
+        n, m = c.shape
+        if ao_packed.numel() != n_pairs(n_pairs(n)):
+            raise ValueError("ao_packed has the wrong length")
+        nout = n_pairs(n_pairs(m))
+        if out is None:
+            out = torch.empty(nout, dtype=torch.float64, device=ao_packed.device)
+        elif out.numel() != nout or out.dtype != torch.float64:
+            raise ValueError("out has the wrong length or type")
+        self._use_torch_stream()
+        self._check(self.lib.cipsi_ijkl_transform_packed_noalloc_dev(
+            self.handle, n, m, _ptr(ao_packed), _ptr(cm), ldc, _ptr(out)),
+            "transform_packed_noalloc_dev")
+        return out
 
     def transform_batch(self, aos, cs, mos=None):
         """A whole scan in one call (cipsi_ijkl_transform_batch_dev): lists of float64 CUDA

@@ -106,6 +106,18 @@ int cipsi_ijkl_transform_labels_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, con
                                     const double *d_c, int ldc, double *d_mo_packed,
                                     int precision, int n_slices);
 
+/* The same FP64 transform as cipsi_ijkl_transform_packed_dev (same kernels, tile plan and
+   bits) for callers that cannot allocate device memory: code under graph capture, tight
+   serving loops, harnesses that forbid hipMalloc.  No counterpart in the reference.  It
+   calls no hipMalloc / hipFree: its workspace is a static 22.2 MB device array, one copy
+   per GPU, held until the process ends, which limits it to n_ao <= 48
+   (CIPSI_ERR_UNSUPPORTED above).  FP64 only.  Asynchronous on the context stream.  Every
+   call on one GPU uses that GPU's single copy, so calls that overlap on one GPU (two
+   streams, two threads, two contexts) race: serialise them. */
+int cipsi_ijkl_transform_packed_noalloc_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo,
+                                            const double *d_ao_packed, const double *d_c,
+                                            int ldc, double *d_mo_packed);
+
 /* transd.f (C1 route, transd.f:85-97,210-218,356-373): label stream -> packed AO block
    on the device.  Returns CIPSI_ERR_LABEL (after synchronising) for a bad label. */
 int cipsi_ijkl_unpack_labels_dev(cipsi_ijkl_ctx *ctx, int n_ao, const double *d_xx,
