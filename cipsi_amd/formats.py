@@ -164,6 +164,57 @@ def write_info(path, n_ao: int, c_square: np.ndarray, h_packed: np.ndarray,
         write_record(f, np.asarray(c_square, dtype="<f8").reshape(-1, order="F").tobytes())
 
 
+def write_info_molecule(path, mol, c_square: np.ndarray, h_packed: np.ndarray,
+                        s_packed: np.ndarray | None = None, enuc: float = 0.0,
+                        n_elec: int = 2, group: str | None = None) -> None:
+    """An info file for a molecule with point-group symmetry (a synth_sym.Molecule):
+    record 1 holds the header and the basis and symmetry tables in the order
+    rondof.f:30-38 reads them -- shells (ktype, kstart, kloc, katom, kng), one primitive
+    per shell, two atoms on the z axis, the image shell of every shell under each of the
+    nt operations (1-based), the p / d / f transformation matrices (diagonal, 3x3, 6x6,
+    10x10 per operation), AO labels, iso, invt, enuc, ante; records 2-7 as write_info."""
+    n = mol.n_ao
+    nsh = mol.nshell
+    nt = mol.nt
+    grp = group or {"C2v": "CNV", "D2h": "DNH"}[mol.group]
+    head = _i4(*([0] * 19)) + _i4(nt, nsh, n, n, 0) + _ch("rhf", 8) \
+        + _i4(nsh, n_elec, 0, 1, n_elec // 2, n_elec // 2, 2) + _ch(grp, 8) + _i4(2) \
+        + _r8(*([0.0] * 10))
+    assert len(head) == 56 * 4
+    body = b"".join(_i4(kt, s + 1, mol.kloc[s] + 1, atom + 1, 1)
+                    for s, (atom, kt) in enumerate(mol.shells))
+    body += b"".join(_r8(1.0 + 0.5 * s, 1.0, 1.0, 1.0, 0.0, 0.0) for s in range(nsh))
+    for atom, z in ((0, 1.25), (1, -1.25)):
+        body += _r8(0.0, 0.0, z) + _r8(0.0) + _i4(1, 1, 1) + _r8(float(n_elec) / 2)
+    body += _i4(*[int(mol.newsh[s, g]) + 1 for g in range(nt) for s in range(nsh)])
+    pd, dd = mol.ptr_diag()
+    ptr = np.zeros((nt, 3, 3))
+    dtr = np.zeros((nt, 6, 6))
+    ftr = np.zeros((nt, 10, 10))
+    for g in range(nt):
+        ptr[g] = np.diag(pd[:, g])
+        dtr[g] = np.diag(dd[:, g])
+        ftr[g] = np.eye(10)
+    # Fortran order: ptr(i, 3(g-1)+c) = M_g[i][c], i fastest
+    body += _r8(*[ptr[g, i, c] for g in range(nt) for c in range(3) for i in range(3)])
+    body += _r8(*[dtr[g, i, c] for g in range(nt) for c in range(6) for i in range(6)])
+    body += _r8(*[ftr[g, i, c] for g in range(nt) for c in range(10) for i in range(10)])
+    body += b"".join(_ch("AT%d" % (mol.shells[mol.ishell[a]][0] + 1), 4) + _ch("%d" % (a + 1), 4)
+                     + _ch("spd"[mol.ktyp[a] - 1] + "%d" % mol.numc[a], 4) for a in range(n))
+    body += _i4(*([1] * (nsh * 12))) + _i4(*([0] * 48))
+    body += _r8(enuc) + _ch("  pshf", 6)
+    pa = n * (n + 1) // 2
+    if s_packed is None:
+        s_packed = np.eye(n)[np.tril_indices(n)]
+    with open(path, "wb") as f:
+        write_record(f, head + body)
+        write_record(f, np.asarray(s_packed, "<f8").tobytes())
+        write_record(f, np.asarray(h_packed, "<f8").tobytes())
+        for _ in range(3):
+            write_record(f, np.zeros(pa, "<f8").tobytes())
+        write_record(f, np.asarray(c_square, dtype="<f8").reshape(-1, order="F").tobytes())
+
+
 # --------------------------------------------------------------------------- namelists
 def ijfil_namelist(prefix: str = "") -> str:
     return "&ijfil prefix='%s' /\n" % prefix if prefix else "&ijfil /\n"

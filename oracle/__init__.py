@@ -5,7 +5,9 @@ cpu_baseline leg of bench.py, as the checker or as the timed host-core baseline.
 The product (cipsi_amd) never imports it.
 
 Each function restates the reference routine named in ijkl_oracle.c
-(CIPSI_sources/ijkl/transd.f, tpqkld.f, tijkld.f, tmono.f) for the C1 case.
+(CIPSI_sources/ijkl/transd.f, tpqkld.f, tijkld.f, tmono.f) for the C1 case, and in
+ijkl_sym_oracle.c (otodon.f, rotat2.f, vector.f, itijkl.f, transd.f, tpqkld.f, tijkld.f)
+for molecules with point-group symmetry.
 """
 from __future__ import annotations
 
@@ -211,3 +213,109 @@ def tijkld_rows(h, c, ilk_list, nthreads=1, out=None):
     lib().or_tijkld_c1_omp_list(n, m, _d(h), _d(c), ldc, _d(out), lst.ctypes.data_as(_lp),
                                 lst.size, nthreads)
     return out
+
+
+# ----------------------------------------------------------------------------- symmetry
+class SymPrep:
+    """State of the reference ijkl after rondof / otodon / rotat2 / vector for a molecule
+    with point-group symmetry (ijkl_sym_oracle.c: or_sym_prepare)."""
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def sym_prepare(mol_tables, s_packed, h_packed, c_square, noc, nvecf=None, ygel=None,
+                trec=1e-5, tdeg=1e-6):
+    """otodon.f:320-353 + rotat2.f:50-423 + vector.f:17-46.  mol_tables: dict with norb,
+    nsym, its (nsym x nsym), jsym (nsym x nt), nt, ktype, kloc (1-based), newsh (nshell x
+    nt, 1-based), iptr (3 x nt), idtr (6 x nt), iftr (10 x nt).  Returns a SymPrep, or
+    raises ValueError(tag) where rotat2 stops with an error tag."""
+    L = lib()
+    ip = ctypes.POINTER(ctypes.c_int)
+    f = L.or_sym_prepare
+    f.restype = ctypes.c_int
+    t = mol_tables
+    n, nsym, nt = t["norb"], t["nsym"], t["nt"]
+    nshell = len(t["ktype"])
+    nvecf = n if nvecf is None else nvecf
+    yg = np.zeros(n, dtype=np.uint8) if ygel is None else np.ascontiguousarray(ygel, np.uint8)
+    its = _i32(np.asarray(t["its"]).T.ravel())          # column-major its(i,j)
+    jsym = _i32(np.asarray(t["jsym"]).T.ravel())
+    newsh = _i32(np.asarray(t["newsh"]).T.ravel())
+    iptr = _i32(np.asarray(t["iptr"]).T.ravel())
+    idtr = _i32(np.asarray(t["idtr"]).T.ravel())
+    iftr = _i32(np.asarray(t["iftr"]).T.ravel())
+    ktype, kloc = _i32(t["ktype"]), _i32(t["kloc"])
+    c_in = np.asfortranarray(c_square, dtype=np.float64)
+    c_out = np.zeros((n, n), order="F")
+    itsyv = np.zeros(n, dtype=np.int32)
+    isymp = np.zeros(nsym, dtype=np.int32)
+    nsymp = ctypes.c_int()
+    ivecsy = np.zeros(nsym * n, dtype=np.int32)
+    nbp = np.zeros(nsym, dtype=np.int32)
+    nbo = np.zeros(nsym, dtype=np.int32)
+    ind = np.zeros(nsym + 1, dtype=np.int32)
+    cpk = np.zeros(n * n)
+    p = lambda a: a.ctypes.data_as(ip)
+    rc = f(n, nsym, p(its), p(jsym), nt, nshell, p(ktype), p(kloc), p(newsh), p(iptr), p(idtr),
+           p(iftr), _d(_f64(s_packed)), _d(_f64(h_packed)), _d(c_in), nvecf,
+           yg.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), noc, ctypes.c_double(trec),
+           ctypes.c_double(tdeg), _d(c_out), p(itsyv), p(isymp), ctypes.byref(nsymp), p(ivecsy),
+           p(nbp), p(nbo), p(ind), _d(cpk))
+    if rc != 0:
+        raise ValueError(rc)
+    r = SymPrep()
+    r.norb, r.nsym, r.nt, r.nvec = n, nsym, nt, nvecf
+    r.its = np.asarray(t["its"], dtype=np.int32)
+    r.c = c_out
+    r.itsyv = itsyv[:nvecf].copy()
+    r.isymp = isymp
+    r.nsymp = nsymp.value
+    r.ivecsy = ivecsy.reshape(nsym, n)
+    r.nbp, r.nbo, r.ind = nbp, nbo, ind
+    r.c_packed = cpk[:int(ind[nsym]) - 1].copy()
+    return r
+
+
+def sym_itijkl(prep, ao_tables, xx, ix, out_cap=None):
+    """itijkl.f:42-298 with transd / tpqkld / tijkld in their symmetric forms.  ao_tables:
+    dict with ishell, kloc, ktyp, numc (per AO, 1-based shell and kloc), newsh (nshell x nt,
+    1-based), iptr, idtr, iftr.  Returns (mo_stream, ipair (npaire x 2), nad, ktypsy)."""
+    L = lib()
+    ip = ctypes.POINTER(ctypes.c_int)
+    f = L.or_sym_itijkl
+    f.restype = ctypes.c_int
+    t = ao_tables
+    n, nsym, nt = prep.norb, prep.nsym, prep.nt
+    nshell = np.asarray(t["newsh"]).shape[0]
+    its = _i32(prep.its.T.ravel())
+    newsh = _i32(np.asarray(t["newsh"]).T.ravel())
+    iptr = _i32(np.asarray(t["iptr"]).T.ravel())
+    idtr = _i32(np.asarray(t["idtr"]).T.ravel())
+    iftr = _i32(np.asarray(t["iftr"]).T.ravel())
+    ishell, kloc, ktyp, numc = (_i32(t[k]) for k in ("ishell", "kloc", "ktyp", "numc"))
+    ivecsy = _i32(prep.ivecsy.ravel())
+    if out_cap is None:
+        pm = n * (n + 1) // 2
+        out_cap = pm * (pm + 1) // 2
+    out = np.zeros(out_cap)
+    nout = ctypes.c_int64()
+    pair_cap = 4096
+    ipair = np.zeros(2 * pair_cap, dtype=np.int32)
+    npaire = ctypes.c_int()
+    nsyt = min(2000, (nsym * (nsym + 1) // 2) * (nsym * (nsym + 1) // 2 + 1) // 2)
+    nad = np.zeros(nsyt, dtype=np.int32)
+    ktypsy = np.zeros(nsyt, dtype=np.int32)
+    xx = _f64(xx)
+    ix = np.ascontiguousarray(ix, dtype=np.int32)
+    p = lambda a: a.ctypes.data_as(ip)
+    rc = f(n, nsym, p(its), p(_i32(prep.isymp)), p(_i32(prep.nbp)), p(_i32(prep.nbo)),
+           p(_i32(prep.ind)), p(ivecsy), _d(_f64(prep.c_packed)), nt, nshell, p(ishell), p(kloc),
+           p(ktyp), p(numc), p(newsh), p(iptr), p(idtr), p(iftr), _d(xx),
+           ix.ctypes.data_as(_ip), xx.size, _d(out), out.size, ctypes.byref(nout), p(ipair),
+           pair_cap, ctypes.byref(npaire), p(nad), p(ktypsy))
+    if rc != 0:
+        raise RuntimeError("or_sym_itijkl failed")
+    npr = npaire.value
+    return out[:nout.value].copy(), ipair[:2 * npr].reshape(npr, 2).copy(), nad, ktypsy

@@ -10,6 +10,11 @@ The AO integrals are regenerated from the seed by cipsi_amd.synth (exact by
 construction, so identical on every machine); a SHA-256 of them is stored and
 checked when the fixture is used.
 
+Symmetric cases (point groups C2v and D2h, synth_sym) store the reference's whole f50
+stream (all symmetry blocks, in its record order), f25 and ijom bytes; their inputs C
+and h are stored too, the AO integrals and the skeleton label stream are regenerated
+from the seed (exact by construction, SHA-256 checked).
+
 Usage:  python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
@@ -107,11 +112,62 @@ def make_case(name):
           f"({os.path.getsize(path) / 1024:.0f} KiB)")
 
 
+# Point-group symmetry (synth_sym): name -> (group, shells of atom 1, shells of atom 2
+# (None: homonuclear), rank per irrep, seed, occupied MOs, degenerate pair)
+SYM_CASES = {
+    "sym_c2v_n19": ("C2v", [1, 1, 2, 3], [1, 2, 3], 2, 5, 3, False),
+    "sym_c2v_deg_n19": ("C2v", [1, 1, 2, 3], [1, 2, 3], 2, 13, 3, True),
+    "sym_d2h_n20": ("D2h", [1, 1, 2, 3], None, 2, 5, 3, False),
+    "sym_d2h_n24": ("D2h", [1, 2, 2, 3], None, 2, 11, 5, False),
+    "sym_c2v_n30": ("C2v", [1, 1, 2, 2, 3, 3], [1, 2, 2, 3], 3, 17, 6, False),
+    "sym_d2h_n38": ("D2h", [1, 1, 1, 2, 2, 3, 3], None, 2, 19, 7, False),
+}
+
+
+def make_sym_case(name):
+    from cipsi_amd import synth_sym
+    grp, sa, sb, rank, seed, noc, deg = SYM_CASES[name]
+    case = synth_sym.make_sym_case(grp, sa, sb, rank_per_irrep=rank, seed=seed,
+                                   degenerate_pair=deg)
+    mol = case["mol"]
+    with tempfile.TemporaryDirectory(prefix="ijkl_ref_sym_") as wd:
+        formats.write_info_molecule(os.path.join(wd, "info"), mol, case["c_square"],
+                                    case["h_packed"], case["s_packed"], enuc=1.5,
+                                    n_elec=2 * noc)
+        nrec = formats.write_pqrs(os.path.join(wd, "pqrs"), case["xx"], case["ix"])
+        stdin = formats.ijfil_namelist() + formats.ijklin_namelist(nrec)
+        t0 = time.time()
+        res = subprocess.run([REF_BIN], input=stdin.encode(), cwd=wd, capture_output=True)
+        dt = time.time() - t0
+        log = res.stdout.decode(errors="replace") + res.stderr.decode(errors="replace")
+        if res.returncode != 0 or "fin ijkl" not in log:
+            raise RuntimeError("reference ijkl failed:\n" + log[-3000:])
+        recs = formats.read_records(os.path.join(wd, "f50"))
+        with open(os.path.join(wd, "f25"), "rb") as f:
+            f25_bytes = f.read()
+        ijom = formats.read_records(os.path.join(wd, "ijom"))
+    stream = np.concatenate([np.frombuffer(r, "<f8") for r in recs[:-1]])
+    out = dict(group=grp, shells_a=np.array(sa), shells_b=np.array(sb if sb else []),
+               rank=rank, seed=seed, noc=noc, degenerate=deg, nrec=nrec, ref_seconds=dt,
+               c_square=case["c_square"], h_packed=case["h_packed"],
+               ao_sha=sha(case["ao_packed"]), xx_sha=sha(case["xx"]), labels_sha=sha(case["ix"]),
+               f50_stream=stream, n_f50_records=len(recs) - 1, hmo=np.frombuffer(recs[-1], "<f8"),
+               f25_bytes=np.frombuffer(f25_bytes, dtype=np.uint8),
+               ijom_c=np.frombuffer(ijom[0], "<f8"), ijom_s=np.frombuffer(ijom[2], "<f8"))
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: N={mol.n_ao} NREC={nrec} ref {dt:.2f}s f50 records={len(recs) - 1} -> {path} "
+          f"({os.path.getsize(path) / 1024:.0f} KiB)")
+
+
 def main(argv):
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"], check=True)
-    names = argv or list(CASES)
+    names = argv or list(CASES) + list(SYM_CASES)
     for n in names:
-        make_case(n)
+        if n in SYM_CASES:
+            make_sym_case(n)
+        else:
+            make_case(n)
 
 
 if __name__ == "__main__":
