@@ -272,6 +272,10 @@ int cipsi_ijkl_release_workspace(cipsi_ijkl_ctx* ctx) {
     ctx_free(ctx->adj_gh);
     ctx_free(ctx->adj_small);
     ctx_free(ctx->cons);
+    ctx_free(ctx->sym_small);
+    ctx_free(ctx->sym_owner);
+    ctx_free(ctx->sym_ht);
+    ctx_free(ctx->sym_tab);
     for (auto& kv : ctx->tile_tables) ctx_free(kv.second.buf);
     ctx->tile_tables.clear();
     for (auto& b : ctx->retired) ctx_free(b);

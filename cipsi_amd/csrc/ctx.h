@@ -50,6 +50,10 @@ struct cipsi_ijkl_ctx {
     DevBuf adj_gh;   // energy/gradient: scaled pair density Gh[b][a] = m(b) Gamma(a,b)
     DevBuf adj_small;  // energy/gradient: h_MO, hC, reduction partials
     DevBuf cons;     // integral consumers: bad-label count, energy
+    DevBuf sym_small;  // point-group path: stream end, basis tables, touched class quartets
+    DevBuf sym_owner;  // point-group path: last-writer keys of the packed block
+    DevBuf sym_ht;     // point-group path: H^T of every block
+    DevBuf sym_tab;    // point-group path: per-block kernel arguments and item tables
     void* blas = nullptr;  // rocblas_handle, created on first gradient call
     cipsi_ijkl_batch* batch_cache = nullptr;  // plan of the last cipsi_ijkl_transform_batch_dev
     // tile plans (kernel variant index) for the two halves: defaults and per-size plans

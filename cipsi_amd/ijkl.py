@@ -50,8 +50,19 @@ EXPORTS = [
     "cipsi_ijkl_transform_batch_dev",
     "cipsi_ijkl_batch_class_info", "cipsi_ijkl_batch_set_class_plan",
     "cipsi_ijkl_gather_dev", "cipsi_ijkl_fock", "cipsi_ijkl_fock_dev",
-    "cipsi_ijkl_hdig_dev", "cipsi_ijkl_hntd_dev",
+    "cipsi_ijkl_hdig_dev", "cipsi_ijkl_hntd_dev", "cipsi_ijkl_sym_transform_labels",
 ]
+
+
+class SymDesc(ctypes.Structure):
+    """cipsi_ijkl_sym: the COMMON state itijkl reads for a molecule with symmetry."""
+    _fields_ = [("n_ao", ctypes.c_int), ("nsym", ctypes.c_int), ("nt", ctypes.c_int),
+                ("nshell", ctypes.c_int)] + \
+               [(f, ctypes.POINTER(ctypes.c_int)) for f in ("its", "isymp", "nbp", "nbo", "ind",
+                                                            "ivecsy")] + \
+               [("c", ctypes.POINTER(ctypes.c_double))] + \
+               [(f, ctypes.POINTER(ctypes.c_int)) for f in ("ishell", "kloc", "ktyp", "numc",
+                                                            "newsh", "iptr", "idtr", "iftr")]
 
 
 class IjklError(RuntimeError):
@@ -131,6 +142,10 @@ def load_library(path: str = LIB_PATH):
     L.cipsi_ijkl_fock_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]
     L.cipsi_ijkl_hdig_dev.argtypes = [_vp, _c_int, _c_int, _c_i64] + [_vp] * 10
     L.cipsi_ijkl_hntd_dev.argtypes = [_vp, _c_int, _c_int, _c_i64] + [_vp] * 7 + [_c_i64, _vp, _vp, _vp]
+    _i16p = ctypes.POINTER(ctypes.c_int16)
+    L.cipsi_ijkl_sym_transform_labels.argtypes = [_vp, ctypes.POINTER(SymDesc), _vp, _vp, _c_i64,
+                                                  _vp, _c_i64, ctypes.POINTER(_c_i64), _vp, _c_int,
+                                                  ctypes.POINTER(_c_int), _i16p, _i16p]
     _lib = L
     return L
 
@@ -248,6 +263,53 @@ class Context:
             self.handle, n, m, _ptr(ao_packed), _ptr(cm), ldc, _ptr(out)),
             "transform_packed_noalloc_dev")
         return out
+
+    def sym_itijkl(self, tables, xx, ix):
+        """itijkl for a molecule with point-group symmetry (cipsi_ijkl_sym_transform_labels).
+
+        tables: dict of the COMMON state after vector.f (1-based, as the reference holds
+        it): n_ao, nsym, nt, its (nsym x nsym), isymp, nbp, nbo, ind, ivecsy (nsym x n_ao),
+        c (vector.f's packed coefficients), ishell, ktyp, numc (per AO), kloc (per shell),
+        newsh (nshell x nt), iptr (3 x nt), idtr (6 x nt), iftr (10 x nt).  xx, ix: the
+        pshf skeleton label stream (numpy).  Returns (mo_stream, ipair (npaire x 2), nad,
+        ktypsy) -- the f50 stream and the f25 directory of itijkl.f:208-213,291-296."""
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        fort = lambda a: i32(np.asarray(a).T.ravel())      # column-major
+        keep = dict(its=fort(tables["its"]), isymp=i32(tables["isymp"]), nbp=i32(tables["nbp"]),
+                    nbo=i32(tables["nbo"]), ind=i32(tables["ind"]),
+                    ivecsy=i32(np.asarray(tables["ivecsy"]).ravel()),
+                    ishell=i32(tables["ishell"]), kloc=i32(tables["kloc"]), ktyp=i32(tables["ktyp"]),
+                    numc=i32(tables["numc"]), newsh=fort(tables["newsh"]), iptr=fort(tables["iptr"]),
+                    idtr=fort(tables["idtr"]), iftr=fort(tables["iftr"]))
+        cpk = np.ascontiguousarray(tables["c"], dtype=np.float64)
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        nshell = np.asarray(tables["newsh"]).shape[0]
+        desc = SymDesc(int(tables["n_ao"]), int(tables["nsym"]), int(tables["nt"]), nshell,
+                       *[ip(keep[k]) for k in ("its", "isymp", "nbp", "nbo", "ind", "ivecsy")],
+                       cpk.ctypes.data_as(_dp),
+                       *[ip(keep[k]) for k in ("ishell", "kloc", "ktyp", "numc", "newsh", "iptr",
+                                               "idtr", "iftr")])
+        xx = np.ascontiguousarray(xx, dtype=np.float64)
+        ix = np.ascontiguousarray(ix, dtype=np.int32)
+        nsym = int(tables["nsym"])
+        nsyt = min(2000, (nsym * (nsym + 1) // 2) * (nsym * (nsym + 1) // 2 + 1) // 2)
+        count = _c_i64(0)
+        npaire = _c_int(0)
+        max_pairs = 8192
+        ipair = np.zeros(2 * max_pairs, dtype=np.int32)
+        nad = np.zeros(nsyt, dtype=np.int16)
+        ktypsy = np.zeros(nsyt, dtype=np.int16)
+        i16 = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int16))
+        self.lib.cipsi_ijkl_reset_stream(self.handle)
+        args = lambda out, cap: (self.handle, ctypes.byref(desc), _ptr(xx), _ptr(ix), xx.size, out,
+                                 cap, ctypes.byref(count), _ptr(ipair), max_pairs,
+                                 ctypes.byref(npaire), i16(nad), i16(ktypsy))
+        self._check(self.lib.cipsi_ijkl_sym_transform_labels(*args(None, 0)), "sym_transform_labels")
+        out = np.empty(max(count.value, 1))
+        self._check(self.lib.cipsi_ijkl_sym_transform_labels(*args(_ptr(out), out.size)),
+                    "sym_transform_labels")
+        npr = npaire.value
+        return out[:count.value], ipair[:2 * npr].reshape(npr, 2).copy(), nad, ktypsy
 
     def transform_batch(self, aos, cs, mos=None):
         """A whole scan in one call (cipsi_ijkl_transform_batch_dev): lists of float64 CUDA

@@ -118,6 +118,46 @@ int cipsi_ijkl_transform_packed_noalloc_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n
                                             const double *d_ao_packed, const double *d_c,
                                             int ldc, double *d_mo_packed);
 
+/* ---- Point-group symmetry (nsym > 1) ---------------------------------------------
+ * The state itijkl reads from COMMON once ijkl.f has run rondof, otodon, rotat2 and
+ * vector (ijkl.f:186-350): the in-process seam `call itijkl(bijkl,bijkl,ydp)`
+ * (ijkl.f:405) for molecules with symmetry.  All indices 1-based as in the reference;
+ * Fortran arrays column-major.
+ *   n_ao, nsym, nt (group operations), nshell
+ *   its      nsym x nsym product table its(i,j)                 (blank COMMON)
+ *   isymp    class of each irrep, non-decreasing (rotat2.f:114-157)
+ *   nbp,nbo  AOs / MOs of each irrep; ind: nsym+1 offsets into c (vector.f:22-46)
+ *   ivecsy   nsym x n_ao, ivecsy[(irrep-1)*n_ao + ao-1]: index of the AO in
+ *            its irrep's subset, 0 if absent (vector.f:26-28)
+ *   c        /om/ c: per irrep, nbp x nbo column-major from ind(irrep)
+ *   ishell, ktyp, numc   per AO: shell, shell type, component (rondof.f:77-82)
+ *   kloc     per shell: first AO; newsh: nshell x nt image shells (rondof.f:30-38)
+ *   iptr, idtr, iftr     3 x nt, 6 x nt, 10 x nt diagonal signs (rondof.f:100-118)
+ * Groups with degenerate-irrep equivalences (C-inf-v / D-inf-h, ntrsy > 0) are not
+ * handled; every AO must belong to the irreps of one class only (true for diatomics in
+ * C2v and D2h). */
+typedef struct {
+    int n_ao, nsym, nt, nshell;
+    const int *its, *isymp, *nbp, *nbo, *ind, *ivecsy;
+    const double *c;
+    const int *ishell, *kloc, *ktyp, *numc, *newsh, *iptr, *idtr, *iftr;
+} cipsi_ijkl_sym;
+
+/* itijkl.f:42-298 for nsym > 1: the pshf skeleton label stream (transd.f regenerates the
+   symmetry images, transd.f:125-244) -> the f50 MO stream of every symmetry-allowed
+   block, in itijkl's order (class quartets, then irrep quartets), each block in
+   tijkld's order (tijkld.f:31-79), and the block directory written to f25
+   (itijkl.f:208-213,291-296): npaire blocks, ipair[2b], ipair[2b+1] = (nij, nkl),
+   nad / ktypsy over nsyt = min(2000, P(P(nsym))) entries.  mo_count receives the
+   stream length; with mo_stream == NULL only the sizes and directory are computed (the
+   label stream is still unpacked: which class quartets hold integrals decides the
+   directory).  Host pointers, synchronous.  FP64 MFMA. */
+int cipsi_ijkl_sym_transform_labels(cipsi_ijkl_ctx *ctx, const cipsi_ijkl_sym *sym,
+                                    const double *xx, const int32_t *labels, int64_t n_entries,
+                                    double *mo_stream, int64_t mo_capacity, int64_t *mo_count,
+                                    int *ipair, int max_pairs, int *npaire, int16_t *nad,
+                                    int16_t *ktypsy);
+
 /* transd.f (C1 route, transd.f:85-97,210-218,356-373): label stream -> packed AO block
    on the device.  Returns CIPSI_ERR_LABEL (after synchronising) for a bad label. */
 int cipsi_ijkl_unpack_labels_dev(cipsi_ijkl_ctx *ctx, int n_ao, const double *d_xx,
