@@ -1,0 +1,328 @@
+// Host-side preparation of the MO coefficients of a molecule with point-group symmetry,
+// the steps the reference ijkl takes before itijkl (ijkl.f:186-350):
+//
+//   group tables   otodon.f:163-317 with the tables of BLOCKDATA1.f: irreps, their
+//                  product table and characters for the non-degenerate groups
+//   symmetry AOs   otodon.f:320-353: per irrep, which AOs carry its symmetry orbitals
+//   rotat2         rotat2.f:50-423: frozen-column compaction; the symmetry vectors
+//                  normalised with S; classes of irreps with the same AOs, made contiguous;
+//                  levels of degenerate MOs (the MO diagonal of h, tmono.f:37-54); the
+//                  irrep of every level by overlap with the symmetry vectors, projecting
+//                  degenerate levels onto their irreps; normalisation and orthogonality
+//                  checks (error tags 2, 1, -1); MOs reordered occupied-by-irrep then
+//                  virtual-by-irrep
+//   vector         vector.f:17-46: per-irrep packing of C on the irrep's AOs
+//
+// Host FP64 code built without fused multiply-add, summing in the reference's order, so
+// the coefficients equal the reference's bit for bit (its ijom file).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/cipsi_ijkl.h"
+
+namespace {
+
+// Characters chi(irrep, operation) of the groups otodon handles without degeneracies,
+// operations in HONDO's order (the jsym2 / jsym4 / jsym8 tables of BLOCKDATA1.f).
+const int kChi2[2][2] = {{1, 1}, {1, -1}};
+const int kChi4[4][4] = {{1, 1, 1, 1}, {1, 1, -1, -1}, {1, -1, 1, -1}, {1, -1, -1, 1}};
+const int kChi8[8][8] = {
+    {1, 1, 1, 1, 1, 1, 1, 1},     {1, 1, -1, -1, 1, 1, -1, -1}, {1, -1, 1, -1, 1, -1, 1, -1},
+    {1, -1, -1, 1, 1, -1, -1, 1}, {1, 1, 1, 1, -1, -1, -1, -1}, {1, 1, -1, -1, -1, -1, 1, 1},
+    {1, -1, 1, -1, -1, 1, -1, 1}, {1, -1, -1, 1, -1, 1, 1, -1}};
+
+inline int64_t tri(int64_t i) { return i * (i - 1) / 2; }      // 1-based pair offset
+
+struct Packed {                                                 // packed symmetric, 1-based
+    const double* p;
+    double operator()(int i, int j) const { return p[(i >= j ? tri(i) + j : tri(j) + i) - 1]; }
+};
+
+bool same_name(const char* g, const char* name) {
+    int k = 0;
+    for (; name[k]; ++k) {
+        char c = g[k];
+        if (c >= 'a' && c <= 'z') c = (char)(c - 'a' + 'A');
+        if (c != name[k]) return false;
+    }
+    for (; k < 8 && g[k]; ++k)
+        if (g[k] != ' ') return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" int cipsi_ijkl_sym_group(const char* group, int naxis, int nt, int* nsym, int* its,
+                                    int* jsym) {
+    if (!group || !nsym || !its || !jsym) return CIPSI_ERR_ARG;
+    int ns = 0;
+    if (same_name(group, "CINFV") || same_name(group, "DINFH")) return CIPSI_ERR_UNSUPPORTED;
+    if (same_name(group, "C1")) ns = 1;
+    else {
+        const bool cs_ci = same_name(group, "CS") || same_name(group, "CI");
+        if (naxis != 2 && !cs_ci) return CIPSI_ERR_UNSUPPORTED;       // otodon.f:276-280
+        if (same_name(group, "CN") || cs_ci) ns = 2;
+        else if (same_name(group, "DN") || same_name(group, "CNV") || same_name(group, "CNH")) ns = 4;
+        else if (same_name(group, "DNH")) ns = 8;
+        else return CIPSI_ERR_ARG;                                      // otodon.f:314-316
+    }
+    if (nt != ns) return CIPSI_ERR_ARG;             // one operation per irrep in these groups
+    *nsym = ns;
+    for (int i = 0; i < ns; ++i)
+        for (int j = 0; j < ns; ++j) {
+            its[i + j * ns] = (i ^ j) + 1;          // the product tables its2 / its4 / its8
+            const int c = ns == 1 ? 1 : ns == 2 ? kChi2[i][j] : ns == 4 ? kChi4[i][j] : kChi8[i][j];
+            jsym[i + j * ns] = c;
+        }
+    return CIPSI_OK;
+}
+
+extern "C" int cipsi_ijkl_sym_prepare(int n_ao, int nsym, const int* its, const int* jsym, int nt,
+                                      int nshell, const int* ktype, const int* kloc,
+                                      const int* newsh, const int* iptr, const int* idtr,
+                                      const int* iftr, const double* s_packed,
+                                      const double* h_packed, const double* c_square, int nvecf,
+                                      const int* ygel, int noc, double trec, double tdeg,
+                                      double* c_out, int* itsyv, int* isymp, int* nsymp,
+                                      int* ivecsy, int* nbp, int* nbo, int* ind, double* c_packed,
+                                      int* tag) {
+    (void)its;
+    if (n_ao < 1 || nsym < 2 || nsym > 8 || nt < 1 || nshell < 1 || nvecf < 1 || nvecf > n_ao ||
+        noc < 0 || !jsym || !ktype || !kloc || !newsh || !iptr || !idtr || !iftr || !h_packed ||
+        !c_square || !c_out || !itsyv || !isymp || !nsymp || !ivecsy || !nbp || !nbo || !ind ||
+        !c_packed)
+        return CIPSI_ERR_ARG;
+    const int n = n_ao;
+    const int64_t nn = (int64_t)n * n;
+    if (tag) *tag = 0;
+    // ---- otodon.f:320-353: per irrep, AOs of its symmetry orbitals (signed SO number)
+    std::vector<int> vs((size_t)nsym * n, 0);
+    {
+        int flat = 0;
+        for (int ir = 0; ir < nsym; ++ir) {
+            int so = 0;
+            for (int sh = 0; sh < nshell; ++sh) {
+                const int kt = ktype[sh];
+                if (kt < 1 || kt > 4) return CIPSI_ERR_ARG;
+                for (int comp = 0; comp < 2 * kt - 1; ++comp, ++flat) {
+                    if (flat >= nsym * n) return CIPSI_ERR_ARG;
+                    if (vs[flat] != 0) continue;
+                    ++so;
+                    for (int g = 0; g < nt; ++g) {
+                        const int img_sh = newsh[sh + g * nshell] - 1;
+                        if (img_sh < 0 || img_sh >= nshell) return CIPSI_ERR_ARG;
+                        const int img = ir * n + kloc[img_sh] - 1 + comp;
+                        if (img < ir * n || img >= (ir + 1) * n) return CIPSI_ERR_ARG;
+                        int sgn = jsym[ir + g * nsym];
+                        if (kt == 2) sgn *= iptr[comp + 3 * g];
+                        else if (kt == 3) sgn *= idtr[comp + 6 * g];
+                        else if (kt == 4) sgn *= iftr[comp + 10 * g];
+                        const int v = so * sgn;
+                        if (vs[img] != 0 && vs[img] != v) {       // the projection vanishes
+                            for (int h = 0; h < nt; ++h)
+                                vs[ir * n + kloc[newsh[sh + h * nshell] - 1] - 1 + comp] = 0;
+                            --so;
+                            break;
+                        }
+                        vs[img] = v;
+                    }
+                }
+            }
+        }
+    }
+    // ---- rotat2.f:50-60: active columns to the front (later columns stay as they were)
+    std::vector<double> c(c_square, c_square + nn);
+    {
+        int act = 0;
+        for (int i = 0; i < nvecf; ++i) {
+            if (ygel && ygel[i]) continue;
+            if (act != i) std::memcpy(&c[(size_t)act * n], &c[(size_t)i * n], sizeof(double) * n);
+            ++act;
+        }
+    }
+    std::vector<double> unit;
+    if (!s_packed) {                                   // yatom: S = 1 (rotat2.f:79-85)
+        unit.assign((size_t)n * (n + 1) / 2, 0.0);
+        for (int i = 1; i <= n; ++i) unit[(size_t)tri(i) + i - 1] = 1.0;
+        s_packed = unit.data();
+    }
+    const Packed S{s_packed}, H{h_packed};
+    // ---- rotat2.f:91-112: symmetry vectors normalised with S; AO membership
+    std::vector<double> vec((size_t)nsym * n);
+    for (size_t k = 0; k < vec.size(); ++k) vec[k] = (double)vs[k];
+    for (int ir = 0; ir < nsym; ++ir) {
+        double* v = &vec[(size_t)ir * n];
+        double norm = 0;
+        for (int i = 1; i <= n; ++i)
+            for (int j = 1; j <= n; ++j) norm = norm + v[i - 1] * S(i, j) * v[j - 1];
+        if (norm < 1.e-8) norm = 1.0;
+        norm = 1.0 / std::sqrt(norm);
+        for (int i = 0; i < n; ++i) v[i] = v[i] * norm;
+        for (int i = 0; i < n; ++i) vs[(size_t)ir * n + i] = std::fabs(v[i]) < 1.e-8 ? 0 : ir + 1;
+    }
+    // ---- rotat2.f:114-130: classes of irreps with the same AOs
+    int ncls = 1;
+    isymp[0] = 1;
+    for (int ir = 1; ir < nsym; ++ir) {
+        int found = -1;
+        for (int jr = 0; jr < ir && found < 0; ++jr) {
+            bool same = true;
+            for (int a = 0; a < n && same; ++a)
+                same = (vs[(size_t)jr * n + a] == 0) == (vs[(size_t)ir * n + a] == 0);
+            if (same) found = jr;
+        }
+        isymp[ir] = found >= 0 ? isymp[found] : ++ncls;
+    }
+    // ---- rotat2.f:131-157: irreps moved so that each class is contiguous (its unchanged)
+    std::vector<int> mo_irrep((size_t)nvecf, 0);       // itsyv of the namelist: 0
+    {
+        int pos = 0;
+        for (int cl = 1; cl <= ncls; ++cl)
+            for (int ir = cl - 1; ir < nsym; ++ir) {
+                if (isymp[ir] != cl) continue;
+                const int dst = pos++;
+                if (ir == dst) continue;
+                isymp[ir] = isymp[dst];
+                isymp[dst] = cl;
+                for (int& m : mo_irrep) {
+                    if (m == ir + 1) m = dst + 1;
+                    else if (m == dst + 1) m = ir + 1;
+                }
+                for (int a = 0; a < n; ++a) {
+                    std::swap(vec[(size_t)ir * n + a], vec[(size_t)dst * n + a]);
+                    std::swap(vs[(size_t)ir * n + a], vs[(size_t)dst * n + a]);
+                }
+            }
+    }
+    // ---- tmono.f:37-54: MO diagonal of h, then rotat2.f:177-203: degenerate levels
+    std::vector<double> hc(nn);
+    for (int i = 0; i < nvecf; ++i)
+        for (int j = 1; j <= n; ++j) {
+            double acc = 0.0;
+            for (int k = 1; k <= n; ++k) acc = acc + c[(size_t)i * n + k - 1] * H(k, j);
+            hc[(size_t)i * n + j - 1] = acc;
+        }
+    std::vector<double> eps(nvecf);
+    for (int i = 0; i < nvecf; ++i) {      // the diagonal element of C^T h C, as tmono sums it
+        double acc = 0.0;
+        for (int k = 0; k < n; ++k) acc = acc + hc[(size_t)i * n + k] * c[(size_t)i * n + k];
+        eps[i] = acc;
+    }
+    std::vector<int> lvl_first, lvl_size;
+    for (int i = 0; i < nvecf;) {
+        int d = 1;
+        while (i + d < nvecf && std::fabs(eps[i] - eps[i + d]) < tdeg) ++d;
+        lvl_first.push_back(i);
+        lvl_size.push_back(d);
+        i += d;
+    }
+    // ---- rotat2.f:210-323: irreps of every level, projection of degenerate levels
+    std::vector<double> sc(nn);            // S c, one column per MO
+    auto apply_s = [&]() {
+        for (int i = 0; i < nvecf; ++i)
+            for (int k = 1; k <= n; ++k) {
+                double acc = 0.0;
+                for (int l = 1; l <= n; ++l) acc = acc + c[(size_t)i * n + l - 1] * S(k, l);
+                sc[(size_t)i * n + k - 1] = acc;
+            }
+    };
+    apply_s();
+    int err_tag = 0;
+    std::vector<double> ov((size_t)nsym * n), proj;
+    for (size_t lv = 0; lv < lvl_first.size() && !err_tag; ++lv) {
+        const int i0 = lvl_first[lv], nd = lvl_size[lv];
+        std::vector<int> hit;
+        for (int ir = 0; ir < nsym; ++ir) {
+            bool any = false;
+            for (int j = 0; j < nd; ++j) {
+                double acc = 0.0;
+                for (int l = 0; l < n; ++l) acc = acc + vec[(size_t)ir * n + l] * sc[(size_t)(i0 + j) * n + l];
+                ov[(size_t)ir * nd + j] = acc;
+                any = any || std::fabs(acc) >= trec;
+            }
+            if (any) hit.push_back(ir);
+        }
+        const int nh = (int)hit.size();
+        if (nh > nd || (nh < nd && nh != 1)) {
+            err_tag = nh < nd ? 2 : 1;
+        } else if (nh < nd) {                            // accidental degeneracy: no rotation
+            for (int j = 0; j < nd; ++j) mo_irrep[i0 + j] = hit[0] + 1;
+        } else {
+            proj.assign((size_t)nd * n, 0.0);
+            for (int t = 0; t < nd; ++t) {
+                const int ir = hit[t];
+                for (int l = 0; l < n; ++l) {
+                    double acc = 0.0;
+                    for (int j = 0; j < nd; ++j)
+                        acc = acc + ov[(size_t)ir * nd + j] * c[(size_t)(i0 + j) * n + l];
+                    proj[(size_t)t * n + l] = acc;
+                }
+                mo_irrep[i0 + t] = ir + 1;
+            }
+            std::memcpy(&c[(size_t)i0 * n], proj.data(), sizeof(double) * nd * n);
+        }
+    }
+    // ---- rotat2.f:327-383: normalisation (tag -1) and orthogonality (tag 1)
+    if (!err_tag) {
+        apply_s();
+        for (int i = 0; i < nvecf && !err_tag; ++i) {
+            double* ci = &c[(size_t)i * n];
+            double* si = &sc[(size_t)i * n];
+            double nrm = 0.0;
+            for (int k = 0; k < n; ++k) nrm = nrm + ci[k] * si[k];
+            double r = std::sqrt(std::fabs(nrm));
+            if (r < 1.e-6) { err_tag = -1; break; }
+            r = 1.0 / r;
+            for (int k = 0; k < n; ++k) {
+                si[k] = si[k] * r;
+                ci[k] = r * ci[k];
+            }
+        }
+        for (int i = 1; i < nvecf && !err_tag; ++i)
+            for (int j = 0; j < i; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < n; ++k) acc = acc + c[(size_t)j * n + k] * sc[(size_t)i * n + k];
+                if (std::fabs(acc) > trec) { err_tag = 1; break; }
+            }
+    }
+    if (err_tag) {
+        if (tag) *tag = err_tag;
+        return CIPSI_ERR_NOT_ORTHONORMAL;
+    }
+    // ---- rotat2.f:386-420: occupied MOs by irrep, then virtual MOs by irrep
+    std::vector<double> out(c);
+    {
+        int n_occ = 0, n_virt = noc;
+        for (int ir = 1; ir <= nsym; ++ir)
+            for (int i = 0; i < nvecf; ++i) {
+                if (mo_irrep[i] != ir) continue;
+                const int k = i + 1 > noc ? n_virt++ : n_occ++;
+                std::memcpy(&out[(size_t)k * n], &c[(size_t)i * n], sizeof(double) * n);
+                itsyv[k] = ir;
+            }
+    }
+    std::memcpy(c_out, out.data(), sizeof(double) * nn);
+    // ---- vector.f:17-46
+    int fill = 0;
+    for (int ir = 0; ir < nsym; ++ir) {
+        ind[ir] = fill + 1;
+        int k = 0;
+        for (int a = 0; a < n; ++a)
+            if (vs[(size_t)ir * n + a] != 0) vs[(size_t)ir * n + a] = ++k;
+        nbp[ir] = k;
+        int m = 0;
+        for (int i = 0; i < nvecf; ++i) {
+            if (itsyv[i] != ir + 1) continue;
+            ++m;
+            for (int a = 0; a < n; ++a)
+                if (vs[(size_t)ir * n + a] != 0) c_packed[fill++] = out[(size_t)i * n + a];
+        }
+        nbo[ir] = m;
+    }
+    ind[nsym] = fill + 1;
+    std::memcpy(ivecsy, vs.data(), sizeof(int) * nsym * n);
+    *nsymp = ncls;
+    return CIPSI_OK;
+}

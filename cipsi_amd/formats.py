@@ -221,7 +221,8 @@ def ijfil_namelist(prefix: str = "") -> str:
 
 
 def ijklin_namelist(nrec: int, nvecf: int = 0, frozen: int = 0, ydp: bool = True,
-                    precision: int | None = None, nslice: int | None = None) -> str:
+                    precision: int | None = None, nslice: int | None = None,
+                    istop: int | None = None, trec: float | None = None) -> str:
     """&ijklin for a C1 run (ijkl.f:28-33); precision / nslice are keys of the GPU host
     only (0 = FP64 MFMA, 1 = split FP16 with nslice slices)."""
     parts = ["ydp=%s" % ("T" if ydp else "F"), "nrec=%d" % nrec, "yauto=T", "chstan=F"]
@@ -233,6 +234,10 @@ def ijklin_namelist(nrec: int, nvecf: int = 0, frozen: int = 0, ydp: bool = True
         parts.append("precision=%d" % precision)
     if nslice is not None:
         parts.append("nslice=%d" % nslice)
+    if istop is not None:
+        parts.append("istop=%d" % istop)
+    if trec is not None:
+        parts.append("trec=%r" % trec)
     return "&ijklin " + ", ".join(parts) + " /\n"
 
 

@@ -18,6 +18,14 @@ module cipsi_ijkl_mod
   integer(c_int), parameter :: CIPSI_PREC_FP64 = 0
   integer(c_int), parameter :: CIPSI_PREC_SPLIT_FP16 = 1
 
+  ! cipsi_ijkl_sym: the COMMON state itijkl reads for a molecule with symmetry (pointers
+  ! to arrays of 1-based values; include/cipsi_ijkl.h documents each member)
+  type, bind(C) :: cipsi_ijkl_sym_t
+    integer(c_int) :: n_ao, nsym, nt, nshell
+    type(c_ptr) :: its, isymp, nbp, nbo, ind, ivecsy, c
+    type(c_ptr) :: ishell, kloc, ktyp, numc, newsh, iptr, idtr, iftr
+  end type
+
   interface
     integer(c_int) function cipsi_ijkl_create(ctx, device) bind(C, name="cipsi_ijkl_create")
       import :: c_int, c_ptr
@@ -116,6 +124,44 @@ module cipsi_ijkl_mod
       type(c_ptr), value :: ctx, ne, nd, trou, part, iorb, ispin, pairs, mo_packed, fock, out
       integer(c_int), value :: n_mo, n_det
       integer(c_int64_t), value :: n_exc, n_pairs
+    end function
+    ! Point-group symmetry: group tables (otodon.f:163-317), the MO preparation of
+    ! otodon / rotat2 / vector (error tags of rotat2 in tag), and the symmetric itijkl.
+    integer(c_int) function cipsi_ijkl_sym_group(group, naxis, nt, nsym, its, jsym) &
+        bind(C, name="cipsi_ijkl_sym_group")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: group(*)
+      integer(c_int), value :: naxis, nt
+      integer(c_int), intent(out) :: nsym, its(*), jsym(*)
+    end function
+    integer(c_int) function cipsi_ijkl_sym_prepare(n_ao, nsym, its, jsym, nt, nshell, ktype, &
+        kloc, newsh, iptr, idtr, iftr, s_packed, h_packed, c_square, nvecf, ygel, noc, trec, &
+        tdeg, c_out, itsyv, isymp, nsymp, ivecsy, nbp, nbo, ind, c_packed, tag) &
+        bind(C, name="cipsi_ijkl_sym_prepare")
+      import :: c_int, c_double
+      integer(c_int), value :: n_ao, nsym, nt, nshell, nvecf, noc
+      integer(c_int), intent(in) :: its(*), jsym(*), ktype(*), kloc(*), newsh(*), iptr(*), &
+                                    idtr(*), iftr(*), ygel(*)
+      real(c_double), intent(in) :: s_packed(*), h_packed(*), c_square(*)
+      real(c_double), value :: trec, tdeg
+      real(c_double), intent(out) :: c_out(*), c_packed(*)
+      integer(c_int), intent(out) :: itsyv(*), isymp(*), nsymp, ivecsy(*), nbp(*), nbo(*), &
+                                     ind(*), tag
+    end function
+    integer(c_int) function cipsi_ijkl_sym_transform_labels(ctx, sym, xx, labels, n_entries, &
+        mo_stream, mo_capacity, mo_count, ipair, max_pairs, npaire, nad, ktypsy) &
+        bind(C, name="cipsi_ijkl_sym_transform_labels")
+      import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t, c_int16_t, cipsi_ijkl_sym_t
+      type(c_ptr), value :: ctx
+      type(cipsi_ijkl_sym_t), intent(in) :: sym
+      real(c_double), intent(in) :: xx(*)
+      integer(c_int32_t), intent(in) :: labels(*)
+      integer(c_int64_t), value :: n_entries, mo_capacity
+      type(c_ptr), value :: mo_stream            ! c_null_ptr: sizes and directory only
+      integer(c_int64_t), intent(out) :: mo_count
+      integer(c_int), intent(out) :: ipair(*), npaire
+      integer(c_int), value :: max_pairs
+      integer(c_int16_t), intent(out) :: nad(*), ktypsy(*)
     end function
   end interface
 

@@ -51,6 +51,7 @@ EXPORTS = [
     "cipsi_ijkl_batch_class_info", "cipsi_ijkl_batch_set_class_plan",
     "cipsi_ijkl_gather_dev", "cipsi_ijkl_fock", "cipsi_ijkl_fock_dev",
     "cipsi_ijkl_hdig_dev", "cipsi_ijkl_hntd_dev", "cipsi_ijkl_sym_transform_labels",
+    "cipsi_ijkl_sym_group", "cipsi_ijkl_sym_prepare",
 ]
 
 
@@ -142,12 +143,78 @@ def load_library(path: str = LIB_PATH):
     L.cipsi_ijkl_fock_dev.argtypes = [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]
     L.cipsi_ijkl_hdig_dev.argtypes = [_vp, _c_int, _c_int, _c_i64] + [_vp] * 10
     L.cipsi_ijkl_hntd_dev.argtypes = [_vp, _c_int, _c_int, _c_i64] + [_vp] * 7 + [_c_i64, _vp, _vp, _vp]
+    _ipi = ctypes.POINTER(_c_int)
+    L.cipsi_ijkl_sym_group.argtypes = [ctypes.c_char_p, _c_int, _c_int, _ipi, _ipi, _ipi]
+    L.cipsi_ijkl_sym_prepare.argtypes = [_c_int, _c_int, _vp, _vp, _c_int, _c_int] + [_vp] * 6 + \
+        [_vp, _vp, _vp, _c_int, _vp, _c_int, ctypes.c_double, ctypes.c_double] + [_vp] * 9 + [_vp]
     _i16p = ctypes.POINTER(ctypes.c_int16)
     L.cipsi_ijkl_sym_transform_labels.argtypes = [_vp, ctypes.POINTER(SymDesc), _vp, _vp, _c_i64,
                                                   _vp, _c_i64, ctypes.POINTER(_c_i64), _vp, _c_int,
                                                   ctypes.POINTER(_c_int), _i16p, _i16p]
     _lib = L
     return L
+
+
+def sym_group(group: str, naxis: int, nt: int):
+    """(nsym, its, jsym) of a non-degenerate point group (cipsi_ijkl_sym_group,
+    otodon.f:163-317); its is nsym x nsym, jsym nsym x nt (irrep x operation)."""
+    L = load_library()
+    ns = _c_int(0)
+    its = np.zeros(64, dtype=np.int32)
+    jsym = np.zeros(64, dtype=np.int32)
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(_c_int))
+    rc = L.cipsi_ijkl_sym_group(group.encode().ljust(8), naxis, nt, ctypes.byref(ns), ip(its), ip(jsym))
+    if rc != 0:
+        raise IjklError(rc, "cipsi_ijkl_sym_group", L.cipsi_ijkl_strerror(rc).decode())
+    k = ns.value
+    return k, its[:k * k].reshape(k, k, order="F").copy(), jsym[:k * nt].reshape(k, nt, order="F").copy()
+
+
+def sym_prepare(tables, s_packed, h_packed, c_square, noc, nvecf=None, ygel=None, trec=1e-5,
+                tdeg=1e-6):
+    """The MO preparation ijkl does before itijkl for a molecule with symmetry
+    (cipsi_ijkl_sym_prepare: otodon + rotat2 + vector), on the host.  tables: norb, nsym,
+    its, jsym (nsym x nt), nt, ktype, kloc, newsh (nshell x nt), iptr, idtr, iftr (1-based
+    values).  s_packed None means S = 1.  Returns a dict with c (the ijom coefficients),
+    itsyv, isymp, nsymp, ivecsy (nsym x n), nbp, nbo, ind, c_packed; raises IjklError
+    (status CIPSI_ERR_NOT_ORTHONORMAL, .tag = rotat2's error tag) where the reference stops."""
+    L = load_library()
+    n, nsym, nt = int(tables["norb"]), int(tables["nsym"]), int(tables["nt"])
+    nvecf = n if nvecf is None else nvecf
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    fort = lambda a: i32(np.asarray(a).T.ravel())
+    its, jsym = fort(tables["its"]), fort(tables["jsym"])
+    ktype, kloc = i32(tables["ktype"]), i32(tables["kloc"])
+    newsh, iptr = fort(tables["newsh"]), fort(tables["iptr"])
+    idtr, iftr = fort(tables["idtr"]), fort(tables["iftr"])
+    sp = None if s_packed is None else np.ascontiguousarray(s_packed, dtype=np.float64)
+    hp = np.ascontiguousarray(h_packed, dtype=np.float64)
+    cs = np.asfortranarray(c_square, dtype=np.float64)
+    yg = None if ygel is None else i32(ygel)
+    c_out = np.zeros((n, n), order="F")
+    itsyv = np.zeros(nvecf, dtype=np.int32)
+    isymp = np.zeros(nsym, dtype=np.int32)
+    nsymp = _c_int(0)
+    ivecsy = np.zeros(nsym * n, dtype=np.int32)
+    nbp = np.zeros(nsym, dtype=np.int32)
+    nbo = np.zeros(nsym, dtype=np.int32)
+    ind = np.zeros(nsym + 1, dtype=np.int32)
+    cpk = np.zeros(n * n)
+    tag = _c_int(0)
+    pv = lambda a: None if a is None else _ptr(a)
+    rc = L.cipsi_ijkl_sym_prepare(n, nsym, _ptr(its), _ptr(jsym), nt, ktype.size, _ptr(ktype),
+                                  _ptr(kloc), _ptr(newsh), _ptr(iptr), _ptr(idtr), _ptr(iftr),
+                                  pv(sp), _ptr(hp), _ptr(cs), nvecf, pv(yg), noc, trec, tdeg,
+                                  _ptr(c_out), _ptr(itsyv), _ptr(isymp), ctypes.byref(nsymp),
+                                  _ptr(ivecsy), _ptr(nbp), _ptr(nbo), _ptr(ind), _ptr(cpk),
+                                  ctypes.byref(tag))
+    if rc != 0:
+        err = IjklError(rc, "cipsi_ijkl_sym_prepare", f"rotat2 error tag {tag.value}")
+        err.tag = tag.value
+        raise err
+    return dict(c=c_out, itsyv=itsyv, isymp=isymp, nsymp=nsymp.value,
+                ivecsy=ivecsy.reshape(nsym, n), nbp=nbp, nbo=nbo, ind=ind,
+                c_packed=cpk[:int(ind[nsym]) - 1].copy())
 
 
 def n_pairs(n: int) -> int:

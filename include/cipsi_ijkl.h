@@ -143,6 +143,37 @@ typedef struct {
     const int *ishell, *kloc, *ktyp, *numc, *newsh, *iptr, *idtr, *iftr;
 } cipsi_ijkl_sym;
 
+/* otodon.f:163-317: irreps, product table its (nsym x nsym, column-major) and characters
+   jsym (nsym x nt, irrep x operation) of a point group without degeneracies, from the
+   info file's group name (8 characters, blank padded, either case) and naxis: C1, CS,
+   CI, CN, CNV, CNH, DN, DNH.  CINFV / DINFH (degenerate irreps, ntrsy > 0) and an axis
+   order other than 2 give CIPSI_ERR_UNSUPPORTED; an unknown name CIPSI_ERR_ARG. */
+int cipsi_ijkl_sym_group(const char *group, int naxis, int nt, int *nsym, int *its, int *jsym);
+
+/* The MO preparation ijkl does before itijkl for nsym > 1, host FP64 (no device work):
+   otodon.f:320-353 (AOs of each irrep's symmetry orbitals), rotat2.f:50-423 (frozen
+   compaction with ygel[0..nvecf), classes of irreps, degenerate levels from the MO
+   diagonal of h, projection onto the irreps, normalisation with S and orthogonality
+   check, MOs reordered occupied-by-irrep then virtual-by-irrep) and vector.f:17-46.
+   Inputs as the info file holds them: basis tables per shell (ktype, kloc, newsh
+   nshell x nt), the diagonal signs iptr / idtr / iftr, S and h packed (s_packed NULL: S
+   = 1, the reference's yatom), C square column-major (record 7).  Outputs: c_out (the
+   coefficients written to ijom, n_ao x n_ao), itsyv (irrep of each of the nvecf MOs),
+   isymp / nsymp (classes), and the COMMON state of cipsi_ijkl_sym: ivecsy, nbp, nbo, ind
+   and c_packed (at most n_ao x n_ao doubles).  Where the reference stops with an error
+   tag (rotat2.f:261-289,347-354,373-380) this returns CIPSI_ERR_NOT_ORTHONORMAL with the
+   tag (2: an MO overlaps too few irreps, 1: too many / not orthogonal, -1: zero norm) in
+   *tag.  Sums run in the reference's order without fused multiply-add, so c_out equals
+   the reference's coefficients bit for bit. */
+int cipsi_ijkl_sym_prepare(int n_ao, int nsym, const int *its, const int *jsym, int nt,
+                           int nshell, const int *ktype, const int *kloc, const int *newsh,
+                           const int *iptr, const int *idtr, const int *iftr,
+                           const double *s_packed, const double *h_packed,
+                           const double *c_square, int nvecf, const int *ygel, int noc,
+                           double trec, double tdeg, double *c_out, int *itsyv, int *isymp,
+                           int *nsymp, int *ivecsy, int *nbp, int *nbo, int *ind,
+                           double *c_packed, int *tag);
+
 /* itijkl.f:42-298 for nsym > 1: the pshf skeleton label stream (transd.f regenerates the
    symmetry images, transd.f:125-244) -> the f50 MO stream of every symmetry-allowed
    block, in itijkl's order (class quartets, then irrep quartets), each block in
