@@ -10,10 +10,12 @@
 //   dE1/dC = 2 h C gamma
 //   dE2/dC(mu,p) = 2 sum_{nu,q} C(nu,q) L[pair(mu,nu)][pair(p,q)],
 //   L = H * Gh,  H[mn][b] = first half (tpqkld.f), Gh[b][a] = m(b) Gamma_ab.
-// The P_A x P_M x P_M product L = H Gh is a plain GEMM (rocBLAS dgemm); everything
-// else is here, in FP64.
+// L is never formed.  With H_b(mu,nu) = H[pair(mu,nu)][b] and Gamma_b(q,p) =
+// Gamma(pair(q,p), b), both symmetric, the sum regroups per MO pair b:
+//   dE2/dC = 2 sum_b m(b) H_b C Gamma_b,
+// two small products per b (2 N^2 M + 2 N M^2 flops) instead of the P_A x P_M x P_M
+// product (2 P_A P_M^2).  All of it is FP64, on the matrix cores.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <vector>
 
@@ -28,16 +30,6 @@ namespace {
 __device__ inline double pair_mult(int64_t a) {
     const int64_t p = pair_row(a);
     return (a - pair0(p, 0) == p) ? 1.0 : 2.0;
-}
-
-// Gh[b][a] = m(b) Gamma(a, b), full P_M x P_M row-major.
-__global__ void k_gamma_hat(const double* __restrict__ gp, int64_t pm, double* __restrict__ gh) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < pm * pm; i += stride) {
-        const int64_t b = i / pm, a = i - b * pm;
-        gh[i] = pair_mult(b) * gp[pairsym(a, b)];
-    }
 }
 
 // Compensated (double-double) accumulation for the energy: error-free two-sum of the
@@ -121,64 +113,243 @@ __global__ void k_hc(const double* __restrict__ hp, const double* __restrict__ c
     hc[i] = s;
 }
 
-// grad(mu, p) = 2 sum_q hc(mu,q) gamma(q,p) + 2 sum_{nu,q} C(nu,q) L[pair(mu,nu)][pair(p,q)]
-// Row r = pair(mu, nu) of L, read as the symmetric M x M matrix M_r(p, q) =
-// L[r][pair(p,q)], enters grad(mu, .) as M_r C(nu, .)^T.  One work-group per (mu, chunk
-// of kGradNu values of nu) streams those rows once each: thread p forms sum_q M_r(p,q)
-// C(nu,q), reading its own contiguous piece (q <= p) and then, coalesced with the other
-// threads, the column below the diagonal (q > p).  C(nu, .) is staged in LDS.  Partial
-// sums per chunk go to part[chunk][mu][p]; k_grad_sum adds them in chunk order (so the
-// result is the same on every run) together with the one-electron term.
-constexpr int kGradNu = 8;
+// ---------------------------------------------------------------- dE2/dC on the matrix cores
+// dE2/dC = 2 sum_b m(b) H_b C Gamma_b.  One wave per work item: a chunk of consecutive MO
+// pairs b and one block of 16 AO rows mu (items are numbered chunk-major, so the waves of
+// a work-group and of neighbouring work-groups share their matrices b in cache).  For
+// each b of the chunk:
+//   stage 1  U(q, mu) = sum_nu C(nu, q) H_b(nu, mu)               (U = (H_b C)^T)
+//            A operand C from its zero-padded copy (16-column blocks), B operand H_b
+//            from the packed row b of H^T; one accumulator per 16 q.
+//   stage 2  G(mu, p) += m(b) sum_q U(q, mu) Gamma_b(q, p)
+//            In the v_mfma_f64_16x16x4f64 D layout lane (lm, lg) holds element j of
+//            block qb as U(16 qb + lg + 4j, 16 mb + lm): exactly A[m = mu][k = q] of the
+//            K step (qb, j), so U stays in registers.  The B operand Gamma_b(q, p) is
+//            read from row b of the expanded pair density (a packed symmetric M x M).
+// BP matrices b are taken per pass and share stage 1's C loads.  m(b) is 1 or 2, so the
+// scaling of U is exact.  After the chunk the wave writes G (its 16 rows mu, every p)
+// to part[chunk][mu][p]; k_grad_reduce adds the chunks in a fixed order, so the
+// gradient is the same on every run.
+// All operand loads are buffer loads over one matrix row (H_b, Gamma_b) or over the
+// padded C: an index outside the row -- pair(nu, mu) with nu or mu >= N, pair(q, p) with
+// q or p >= M -- is past the row's last element, and the hardware returns 0 for it.  A
+// matrix b past the chunk's end gets an empty range and contributes nothing.
+// Operands are prefetched two K groups ahead into register rings, with scheduling fences
+// between the groups so that the compiler keeps the rings at their size.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
 
-__global__ __launch_bounds__(256) void k_grad_rows(const double* __restrict__ L, int64_t pm,
-                                                   const double* __restrict__ c, int ldc, int n,
-                                                   int m, double* __restrict__ part) {
-    __shared__ double cs[256];
-    const int mu = blockIdx.x, ch = blockIdx.y;
-    const int p = threadIdx.x;
-    const int nu0 = ch * kGradNu, nu1 = min(n, nu0 + kGradNu);
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};               // q mod 4: independent FMA chains
-    for (int nu = nu0; nu < nu1; ++nu) {
-        __syncthreads();                               // previous nu's reads of cs done
-        if (p < m) cs[p] = c[(int64_t)p * ldc + nu];
-        __syncthreads();
-        if (p < m) {
-            const double* __restrict__ row = L + pairsym(mu, nu) * pm;
-            const double* __restrict__ rp = row + (int64_t)p * (p + 1) / 2;
-            int q = 0;
-            for (; q + 3 <= p; q += 4) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[e] += rp[q + e] * cs[q + e];
-            }
-            for (; q <= p; ++q) acc[q & 3] += rp[q] * cs[q];
-            for (q = p + 1; q + 3 < m; q += 4) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    acc[(q + e) & 3] += row[(int64_t)(q + e) * (q + e + 1) / 2 + p] * cs[q + e];
-            }
-            for (; q < m; ++q) acc[q & 3] += row[(int64_t)q * (q + 1) / 2 + p] * cs[q];
-        }
-    }
-    if (p < m) part[((int64_t)ch * n + mu) * m + p] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+__device__ __forceinline__ Rsrc row_rsrc(const double* p, int64_t elems) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(elems * 8), 0x00020000);
+}
+__device__ __forceinline__ double bload(Rsrc r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ unsigned packed_off(unsigned hi, unsigned lo) {
+    return (((hi * (hi + 1)) >> 1) + lo) * 8;
 }
 
-__global__ __launch_bounds__(256) void k_grad_sum(const double* __restrict__ part, int nch,
-                                                  const double* __restrict__ hc,
-                                                  const double* __restrict__ gamma, int n, int m,
-                                                  double* __restrict__ grad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n * m) return;
-    const int mu = i / m, p = i - mu * m;
-    double s = 0.0;
-    for (int ch = 0; ch < nch; ++ch) s += part[((int64_t)ch * n + mu) * m + p];
-    for (int q = 0; q < m; ++q) s += hc[(int64_t)q * n + mu] * gamma[(int64_t)q * m + p];
-    grad[(int64_t)p * n + mu] = 2.0 * s;
+template <int QB, int BP>
+__global__ __launch_bounds__(256) void k_grad_mfma(const double* __restrict__ ht, int64_t pa,
+                                                   const double* __restrict__ gam, int64_t pm,
+                                                   const double* __restrict__ cr, int n, int m,
+                                                   int nmb, int chunk, int64_t nitems,
+                                                   double* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lm = lane & 15, lg = lane >> 4;
+    const int64_t item = xcd_item(blockIdx.x) * 4 + w;
+    if (item >= nitems) return;                        // whole wave; no barriers follow
+    const int64_t ch = item / nmb;
+    const int mb = (int)(item - ch * nmb);
+    const int64_t b0 = ch * chunk, b1 = b0 + chunk < pm ? b0 + chunk : pm;
+    const unsigned mu = 16 * mb + lm;
+    const int nk = 4 * nmb;                            // K steps over nu (rows padded to 16)
+    // padded C: block qb holds kCRows rows of 16 doubles (32 KiB)
+    const Rsrc crs = row_rsrc(cr, (int64_t)QB * kCRows * 16);
+    const unsigned coff = (lg * 16 + lm) * 8;          // C(4k + lg, 16 qb + lm) - k, qb terms
+    f64x4 G[QB];
+#pragma unroll
+    for (int i = 0; i < QB; ++i) G[i] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t b = b0; b < b1; b += BP) {
+        Rsrc hr[BP], gr[BP];
+        double mult[BP];
+#pragma unroll
+        for (int e = 0; e < BP; ++e) {
+            const bool ok = b + e < b1;
+            const int64_t bb = ok ? b + e : b;
+            hr[e] = row_rsrc(ht + bb * pa, ok ? pa : 0);
+            gr[e] = row_rsrc(gam + bb * pm, ok ? pm : 0);
+            mult[e] = ok ? pair_mult(bb) : 0.0;
+        }
+        f64x4 U[BP][QB];
+#pragma unroll
+        for (int e = 0; e < BP; ++e)
+#pragma unroll
+            for (int i = 0; i < QB; ++i) U[e][i] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+        // stage 1: K step k covers nu = 4k + {0..3}; lane (lm, lg) supplies nu = 4k + lg.
+        // A step past the last one loads only zeros or unused values (its offsets are past
+        // the rows), so the two-step loop needs no guard.
+        auto ld1 = [&](int k, double* av, double* hv) {
+            const unsigned nu = 4 * k + lg;
+            const unsigned ho = nu > mu ? packed_off(nu, mu) : packed_off(mu, nu);
+#pragma unroll
+            for (int e = 0; e < BP; ++e) hv[e] = bload(hr[e], ho, 0);
+#pragma unroll
+            for (int qb = 0; qb < QB; ++qb) av[qb] = bload(crs, coff + k * 512, qb * (kCRows * 128));
+        };
+        auto mm1 = [&](const double* av, const double* hv) {
+#pragma unroll
+            for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+                for (int e = 0; e < BP; ++e)
+                    U[e][qb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[qb], hv[e], U[e][qb], 0, 0, 0);
+        };
+        double a0[QB], a1[QB], h0[BP], h1[BP];
+        ld1(0, a0, h0);
+        for (int k = 0; k < nk; k += 2) {
+            ld1(k + 1, a1, h1);
+            __builtin_amdgcn_sched_barrier(0);
+            mm1(a0, h0);
+            __builtin_amdgcn_sched_barrier(0);
+            ld1(k + 2, a0, h0);
+            __builtin_amdgcn_sched_barrier(0);
+            mm1(a1, h1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+
+        // stage 2: group g = (e, qb, j) is the K step q = 16 qb + 4 j + {0..3} of matrix
+        // e; lane (lm, lg) loads Gamma_b(16 qb + 4 j + lg, 16 pb + lm) for every pb.
+        constexpr int NG = BP * QB * 4;
+        constexpr int RD = 3;
+        double ring[RD][QB];
+        auto ld2 = [&](int g, double* gv) {
+            const int e = g / (QB * 4), qb = (g >> 2) % QB, j = g & 3;
+            // The offsets depend only on the lane; an opaque copy of the lane index per
+            // group keeps the compiler from hoisting all QB * 4 * QB of them out of the
+            // loop over b (they would stay live in registers).
+            unsigned lgv = lg, lmv = lm;
+            asm volatile("" : "+v"(lgv), "+v"(lmv));
+            const unsigned q = 16 * qb + 4 * j + lgv;
+#pragma unroll
+            for (int pb = 0; pb < QB; ++pb) {
+                const unsigned p = 16 * pb + lmv;
+                const unsigned off = pb < qb ? packed_off(q, p)
+                                   : pb > qb ? packed_off(p, q)
+                                             : (q >= p ? packed_off(q, p) : packed_off(p, q));
+                gv[pb] = bload(gr[e], off, 0);
+            }
+        };
+        ld2(0, ring[0]);
+        if (NG > 1) ld2(1, ring[1]);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            if (g + 2 < NG) ld2(g + 2, ring[(g + 2) % RD]);
+            __builtin_amdgcn_sched_barrier(0);
+            const int e = g / (QB * 4), qb = (g >> 2) % QB, j = g & 3;
+            const double a = U[e][qb][j] * mult[e];
+#pragma unroll
+            for (int pb = 0; pb < QB; ++pb)
+                G[pb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ring[g % RD][pb], G[pb], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // lane (lm, lg), element j of block pb: G(16 mb + lg + 4j, 16 pb + lm)
+    double* __restrict__ dst = part + ch * (int64_t)n * m;
+#pragma unroll
+    for (int pb = 0; pb < QB; ++pb) {
+        const int p = 16 * pb + lm;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = 16 * mb + lg + 4 * j;
+            if (r < n && p < m) dst[(int64_t)r * m + p] = G[pb][j];
+        }
+    }
+}
+
+// grad(mu, p) = 2 (sum_chunks part[chunk][mu][p] + sum_q hc(mu, q) gamma(q, p)), written
+// column-major (ld n).  64 outputs per work-group; its four waves take the chunks
+// congruent to 0..3 mod 4, each in chunk order, and the four sums are added in a fixed
+// order.
+__global__ __launch_bounds__(256) void k_grad_reduce(const double* __restrict__ part, int64_t nch,
+                                                     const double* __restrict__ hc,
+                                                     const double* __restrict__ gamma, int n,
+                                                     int m, double* __restrict__ grad) {
+    __shared__ double red[4][64];
+    const int t = threadIdx.x & 63, s = threadIdx.x >> 6;
+    const int64_t nm = (int64_t)n * m;
+    const int64_t o = (int64_t)blockIdx.x * 64 + t;
+    double acc = 0.0;
+    if (o < nm)
+        for (int64_t c = s; c < nch; c += 4) acc += part[c * nm + o];
+    red[s][t] = acc;
+    __syncthreads();
+    if (s != 0 || o >= nm) return;
+    const int mu = (int)(o / m), p = (int)(o - (int64_t)mu * m);
+    double v = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+    for (int q = 0; q < m; ++q) v += hc[(int64_t)q * n + mu] * gamma[(int64_t)q * m + p];
+    grad[(int64_t)p * n + mu] = 2.0 * v;
+}
+
+// Work split of k_grad_mfma: chunks of MO pairs sized so that a launch has about 4096
+// waves (4 per SIMD), at least 2 and at most 64 matrices per chunk, a multiple of the
+// pass width.  Two matrices per pass only while a wave still fits twice on a SIMD
+// (up to 5 blocks of 16 MOs); beyond that the second U set would cost occupancy.
+constexpr int kGradPairMaxQB = 5;
+struct GradPlan {
+    int nmb, qb, bp, chunk;
+    int64_t nch, nitems, grid;
+};
+
+GradPlan grad_plan(int n, int m) {
+    GradPlan g{};
+    g.nmb = (n + 15) / 16;
+    g.qb = (m + 15) / 16;
+    g.bp = g.qb <= kGradPairMaxQB ? 2 : 1;
+    const int64_t pm = npairs(m);
+    int64_t c = (pm * g.nmb + 4095) / 4096;
+    c = c < 2 ? 2 : (c > 64 ? 64 : c);
+    c = (c + g.bp - 1) / g.bp * g.bp;
+    g.chunk = (int)c;
+    g.nch = (pm + c - 1) / c;
+    g.nitems = g.nch * g.nmb;
+    g.grid = xcd_grid((g.nitems + 3) / 4);
+    return g;
+}
+
+template <int QB>
+hipError_t launch_grad_qb(const GradPlan& g, const double* ht, int64_t pa, const double* gam,
+                          int64_t pm, const double* cr, int n, int m, double* part,
+                          hipStream_t st) {
+    if (g.bp == 2) {
+        if constexpr (QB <= kGradPairMaxQB)
+            hipLaunchKernelGGL((k_grad_mfma<QB, 2>), dim3((unsigned)g.grid), dim3(256), 0, st, ht,
+                               pa, gam, pm, cr, n, m, g.nmb, g.chunk, g.nitems, part);
+        else
+            return hipErrorInvalidValue;
+    } else {
+        hipLaunchKernelGGL((k_grad_mfma<QB, 1>), dim3((unsigned)g.grid), dim3(256), 0, st, ht, pa,
+                           gam, pm, cr, n, m, g.nmb, g.chunk, g.nitems, part);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_grad(const GradPlan& g, const double* ht, int64_t pa, const double* gam,
+                       int64_t pm, const double* cr, int n, int m, double* part, hipStream_t st) {
+    if (g.grid > 0x7fffffff || n > kCRows || npairs(n) * 8 > 0x7fffffff) return hipErrorInvalidValue;
+    switch (g.qb) {
+#define CIPSI_GRAD_CASE(Q) \
+    case Q: return launch_grad_qb<Q>(g, ht, pa, gam, pm, cr, n, m, part, st);
+        CIPSI_GRAD_CASE(1) CIPSI_GRAD_CASE(2) CIPSI_GRAD_CASE(3) CIPSI_GRAD_CASE(4)
+        CIPSI_GRAD_CASE(5) CIPSI_GRAD_CASE(6) CIPSI_GRAD_CASE(7) CIPSI_GRAD_CASE(8)
+        CIPSI_GRAD_CASE(9) CIPSI_GRAD_CASE(10) CIPSI_GRAD_CASE(11) CIPSI_GRAD_CASE(12)
+        CIPSI_GRAD_CASE(13) CIPSI_GRAD_CASE(14) CIPSI_GRAD_CASE(15) CIPSI_GRAD_CASE(16)
+#undef CIPSI_GRAD_CASE
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
-
-int ctx_rocblas(cipsi_ijkl_ctx* ctx);
 
 static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, const double* d_hp,
                        const double* d_c, int ldc, const double* d_gamma, const double* d_Gamma,
@@ -188,9 +359,9 @@ static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, co
     if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->h, sizeof(double) * pa * pm))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->adj_g, sizeof(double) * nout))) return rc;
-    const int nch = (n + kGradNu - 1) / kGradNu;
+    const GradPlan gp = grad_plan(n, m);
     if ((rc = ctx_alloc(ctx, ctx->adj_small,
-                        sizeof(double) * (pm + (2 + (size_t)nch) * n * m + 4096))))
+                        sizeof(double) * (pm + (1 + (size_t)gp.nch) * n * m + 4096))))
         return rc;
     double* vsq = (double*)ctx->vsq.ptr;
     double* h = (double*)ctx->h.ptr;
@@ -241,45 +412,26 @@ static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, co
     if (hipGetLastError() != hipSuccess) return CIPSI_ERR_HIP;
     if (!d_grad) return CIPSI_OK;
 
-    // gradient: L = H Gh (rocBLAS, column-major view: L^T = Gh^T H^T), then the contraction
+    // gradient: Gamma expanded to the full P_M x P_M (row b = Gamma_b packed), then
+    // 2 sum_b m(b) H_b C Gamma_b from the rows of H^T still in vsq, and the reduction
     if ((rc = ctx_alloc(ctx, ctx->adj_gh, sizeof(double) * pm * pm))) return rc;
-    double* gh = (double*)ctx->adj_gh.ptr;
-    double* L = vsq;                                   // P_A x P_M fits in the P_A^2 buffer
-    if ((rc = ctx_rocblas(ctx))) return rc;
+    double* gfull = (double*)ctx->adj_gh.ptr;
     {
-        StageTimer tm(ctx, "grad_gemm");
-        hipLaunchKernelGGL(k_gamma_hat, dim3(4096), dim3(256), 0, ctx->stream, d_Gamma, pm, gh);
-        const double one = 1.0, zero = 0.0;
-        if (rocblas_dgemm((rocblas_handle)ctx->blas, rocblas_operation_none,
-                          rocblas_operation_none, (rocblas_int)pm, (rocblas_int)pa,
-                          (rocblas_int)pm, &one, gh, (rocblas_int)pm, h, (rocblas_int)pm, &zero,
-                          L, (rocblas_int)pm) != rocblas_status_success)
+        StageTimer tm(ctx, "grad_gamma");
+        if (launch_expand_packed(d_Gamma, pm, gfull, ctx->stream) != hipSuccess)
+            return CIPSI_ERR_HIP;
+        hipLaunchKernelGGL(k_hc, dim3((n * m + 255) / 256), dim3(256), 0, ctx->stream, d_hp, d_c,
+                           ldc, n, m, hc);
+    }
+    {
+        StageTimer tm(ctx, "grad_mfma");
+        if (launch_grad(gp, vsq, pa, gfull, pm, cr, n, m, gpart, ctx->stream) != hipSuccess)
             return CIPSI_ERR_HIP;
     }
-    StageTimer tm(ctx, "grad_contract");
-    hipLaunchKernelGGL(k_hc, dim3((n * m + 255) / 256), dim3(256), 0, ctx->stream, d_hp, d_c, ldc,
-                       n, m, hc);
-    hipLaunchKernelGGL(k_grad_rows, dim3(n, nch), dim3(256), 0, ctx->stream, L, pm, d_c, ldc, n, m,
-                       gpart);
-    hipLaunchKernelGGL(k_grad_sum, dim3((n * m + 255) / 256), dim3(256), 0, ctx->stream, gpart,
-                       nch, hc, d_gamma, n, m, d_grad);
+    StageTimer tm(ctx, "grad_reduce");
+    hipLaunchKernelGGL(k_grad_reduce, dim3((unsigned)((n * m + 63) / 64)), dim3(256), 0,
+                       ctx->stream, gpart, gp.nch, hc, d_gamma, n, m, d_grad);
     return hipGetLastError() == hipSuccess ? CIPSI_OK : CIPSI_ERR_HIP;
-}
-
-int ctx_rocblas(cipsi_ijkl_ctx* ctx) {
-    if (!ctx->blas) {
-        rocblas_handle hdl;
-        if (rocblas_create_handle(&hdl) != rocblas_status_success) return CIPSI_ERR_HIP;
-        ctx->blas = hdl;
-    }
-    if (rocblas_set_stream((rocblas_handle)ctx->blas, ctx->stream) != rocblas_status_success)
-        return CIPSI_ERR_HIP;
-    return CIPSI_OK;
-}
-
-void ctx_rocblas_destroy(cipsi_ijkl_ctx* ctx) {
-    if (ctx->blas) (void)rocblas_destroy_handle((rocblas_handle)ctx->blas);
-    ctx->blas = nullptr;
 }
 
 static int check_args(int n, int m, int ldc) {

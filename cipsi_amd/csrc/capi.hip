@@ -290,7 +290,6 @@ int cipsi_ijkl_destroy(cipsi_ijkl_ctx* ctx) {
     cipsi_ijkl_reset_timing(ctx);
     cipsi_ijkl_release_workspace(ctx);
     pin_free(ctx);
-    ctx_rocblas_destroy(ctx);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return CIPSI_OK;

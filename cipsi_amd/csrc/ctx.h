@@ -54,7 +54,6 @@ struct cipsi_ijkl_ctx {
     DevBuf sym_owner;  // point-group path: last-writer keys of the packed block
     DevBuf sym_ht;     // point-group path: H^T of every block
     DevBuf sym_tab;    // point-group path: per-block kernel arguments and item tables
-    void* blas = nullptr;  // rocblas_handle, created on first gradient call
     cipsi_ijkl_batch* batch_cache = nullptr;  // plan of the last cipsi_ijkl_transform_batch_dev
     // tile plans (kernel variant index) for the two halves: defaults and per-size plans
     int var_half1 = 2;
@@ -109,7 +108,6 @@ int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half, int n, int m, int prec
 int noalloc_setup(int device);
 
 // energy / gradient (adjoint.hip)
-void ctx_rocblas_destroy(cipsi_ijkl_ctx* ctx);
 
 // split-FP16 path (half_split.hip)
 bool split_supported(int n_slices);
