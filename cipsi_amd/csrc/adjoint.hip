@@ -51,9 +51,24 @@ __device__ inline void dd_add(DD& a, const DD& b) {
     a.lo += b.lo;
 }
 
+// m(b) for every MO pair b, and the symmetrised one-particle density packed like h_MO:
+// gs[pair0(p, q)] = (gamma(p,q) + gamma(q,p)) / 2 (= gamma(p,q) for a symmetric gamma,
+// exactly), the matrix whose C-derivative term is 2 h C gs.
+__global__ void k_pair_tables(const double* __restrict__ gamma, int m, int64_t pm,
+                              double* __restrict__ mult, double* __restrict__ gs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pm) return;
+    const int64_t p = pair_row(i), q = i - pair0(p, 0);
+    mult[i] = p == q ? 1.0 : 2.0;
+    gs[i] = 0.5 * (gamma[p * m + q] + gamma[q * m + p]);
+}
+
 // Per-block partial sums of w m(a) m(b) Gamma_ab g_ab over the packed index (w = 2 off
 // the diagonal of the pair supermatrix), in double-double; a fixed-order final pass
-// makes the result deterministic.
+// makes the result deterministic.  Block k takes the contiguous range
+// [k L, (k + 1) L) of the packed index, L a multiple of RB; a thread steps through it RB
+// elements at a time and carries its (row a, column b) along, so only its first
+// element is decoded with a square root.
 constexpr int RB = 256;
 __device__ inline void block_reduce_dd(DD v, DD* red) {
     red[threadIdx.x] = v;
@@ -66,13 +81,24 @@ __device__ inline void block_reduce_dd(DD v, DD* red) {
 
 __global__ __launch_bounds__(RB) void k_energy2_partial(const double* __restrict__ g,
                                                         const double* __restrict__ gp,
-                                                        int64_t n, double* __restrict__ part) {
+                                                        const double* __restrict__ mult,
+                                                        int64_t n, int64_t len,
+                                                        double* __restrict__ part) {
     __shared__ DD red[RB];
     DD s{0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
-        const int64_t a = pair_row(i), b = i - pair0(a, 0);
-        const double w = (a == b) ? 1.0 : 2.0;
-        dd_add(s, w * pair_mult(a) * pair_mult(b) * gp[i] * g[i]);
+    const int64_t i0 = (int64_t)blockIdx.x * len + threadIdx.x;
+    const int64_t iend = (int64_t)(blockIdx.x + 1) * len < n ? (int64_t)(blockIdx.x + 1) * len : n;
+    if (i0 < iend) {
+        int64_t a = pair_row(i0), b = i0 - pair0(a, 0);
+        for (int64_t i = i0; i < iend; i += RB) {
+            const double w = (a == b) ? 1.0 : 2.0;
+            dd_add(s, w * mult[a] * mult[b] * gp[i] * g[i]);
+            b += RB;
+            while (b > a) {
+                b -= a + 1;
+                ++a;
+            }
+        }
     }
     block_reduce_dd(s, red);
     if (threadIdx.x == 0) {
@@ -81,7 +107,8 @@ __global__ __launch_bounds__(RB) void k_energy2_partial(const double* __restrict
     }
 }
 
-// energy = sum_pq hmo(pq) gamma(p,q) + 1/2 sum(part); one block.
+// energy = sum_pq hmo(pq) gamma(p,q) + 1/2 sum(part); one block, both sums reduced
+// across its threads in a fixed order.
 __global__ __launch_bounds__(RB) void k_energy_final(const double* __restrict__ part, int np_,
                                                      const double* __restrict__ hmo,
                                                      const double* __restrict__ gamma, int m,
@@ -93,29 +120,22 @@ __global__ __launch_bounds__(RB) void k_energy_final(const double* __restrict__ 
         dd_add(s, hmo[pairsym(p, q)] * gamma[i]);
     }
     block_reduce_dd(s, red);
+    const DD e1 = red[0];
+    __syncthreads();                                   // everyone has read red[0]
+    DD s2{0.0, 0.0};
+    for (int i = threadIdx.x; i < np_; i += RB) dd_add(s2, DD{part[2 * i], part[2 * i + 1]});
+    block_reduce_dd(s2, red);
     if (threadIdx.x == 0) {
-        DD e2{0.0, 0.0};
-        for (int i = 0; i < np_; ++i) dd_add(e2, DD{part[2 * i], part[2 * i + 1]});
-        DD e = red[0];
-        dd_add(e, DD{0.5 * e2.hi, 0.5 * e2.lo});
+        DD e = e1;
+        dd_add(e, DD{0.5 * red[0].hi, 0.5 * red[0].lo});
         *energy = e.hi + e.lo;
     }
 }
 
-// hc(mu, q) = sum_nu h(mu, nu) C(nu, q)   (N x M, column-major, ld n)
-__global__ void k_hc(const double* __restrict__ hp, const double* __restrict__ c, int ldc, int n,
-                     int m, double* __restrict__ hc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n * m) return;
-    const int q = i / n, mu = i - q * n;
-    double s = 0.0;
-    for (int nu = 0; nu < n; ++nu) s += hp[pairsym(mu, nu)] * c[(int64_t)q * ldc + nu];
-    hc[i] = s;
-}
-
 // ---------------------------------------------------------------- dE2/dC on the matrix cores
-// dE2/dC = 2 sum_b m(b) H_b C Gamma_b.  One wave per work item: a chunk of consecutive MO
-// pairs b and one block of 16 AO rows mu (items are numbered chunk-major, so the waves of
+// dE/dC = 2 sum_b m(b) H_b C Gamma_b + 2 h C gs.  The one-electron term is one more
+// matrix b = P_M after the MO pairs, with H_b = h, Gamma_b = gs and m(b) = 1.  One wave
+// per work item: a chunk of consecutive matrices b and one block of 16 AO rows mu (items are numbered chunk-major, so the waves of
 // a work-group and of neighbouring work-groups share their matrices b in cache).  For
 // each b of the chunk:
 //   stage 1  U(q, mu) = sum_nu C(nu, q) H_b(nu, mu)               (U = (H_b C)^T)
@@ -126,8 +146,9 @@ __global__ void k_hc(const double* __restrict__ hp, const double* __restrict__ c
 //            block qb as U(16 qb + lg + 4j, 16 mb + lm): exactly A[m = mu][k = q] of the
 //            K step (qb, j), so U stays in registers.  The B operand Gamma_b(q, p) is
 //            read from row b of the expanded pair density (a packed symmetric M x M).
-// BP matrices b are taken per pass and share stage 1's C loads.  m(b) is 1 or 2, so the
-// scaling of U is exact.  After the chunk the wave writes G (its 16 rows mu, every p)
+// BP matrices b are taken per pass and share stage 1's C loads.  m(b) (1 or 2, so the
+// scaling is exact) multiplies the H_b operand as it is loaded: stage 2 then takes U
+// from the accumulators as it is, with no vector arithmetic on it.  After the chunk the wave writes G (its 16 rows mu, every p)
 // to part[chunk][mu][p]; k_grad_reduce adds the chunks in a fixed order, so the
 // gradient is the same on every run.
 // All operand loads are buffer loads over one matrix row (H_b, Gamma_b) or over the
@@ -135,7 +156,10 @@ __global__ void k_hc(const double* __restrict__ hp, const double* __restrict__ c
 // q or p >= M -- is past the row's last element, and the hardware returns 0 for it.  A
 // matrix b past the chunk's end gets an empty range and contributes nothing.
 // Operands are prefetched two K groups ahead into register rings, with scheduling fences
-// between the groups so that the compiler keeps the rings at their size.
+// between the groups so that the compiler keeps the rings at their size.  The file is
+// compiled with every MFMA operand in the ordinary vector registers
+// (-amdgpu-mfma-vgpr-form): with accumulator registers the compiler copied U back to
+// vector registers on every K step of stage 1.
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 
 __device__ __forceinline__ Rsrc row_rsrc(const double* p, int64_t elems) {
@@ -151,7 +175,9 @@ __device__ __forceinline__ unsigned packed_off(unsigned hi, unsigned lo) {
 template <int QB, int BP>
 __global__ __launch_bounds__(256) void k_grad_mfma(const double* __restrict__ ht, int64_t pa,
                                                    const double* __restrict__ gam, int64_t pm,
-                                                   const double* __restrict__ cr, int n, int m,
+                                                   const double* __restrict__ cr,
+                                                   const double* __restrict__ h1,
+                                                   const double* __restrict__ g1, int n, int m,
                                                    int nmb, int chunk, int64_t nitems,
                                                    double* __restrict__ part) {
     const int lane = threadIdx.x & 63;
@@ -161,12 +187,18 @@ __global__ __launch_bounds__(256) void k_grad_mfma(const double* __restrict__ ht
     if (item >= nitems) return;                        // whole wave; no barriers follow
     const int64_t ch = item / nmb;
     const int mb = (int)(item - ch * nmb);
-    const int64_t b0 = ch * chunk, b1 = b0 + chunk < pm ? b0 + chunk : pm;
+    const int64_t b0 = ch * chunk, b1 = b0 + chunk < pm + 1 ? b0 + chunk : pm + 1;
     const unsigned mu = 16 * mb + lm;
     const int nk = 4 * nmb;                            // K steps over nu (rows padded to 16)
     // padded C: block qb holds kCRows rows of 16 doubles (32 KiB)
     const Rsrc crs = row_rsrc(cr, (int64_t)QB * kCRows * 16);
     const unsigned coff = (lg * 16 + lm) * 8;          // C(4k + lg, 16 qb + lm) - k, qb terms
+    unsigned ap[QB];                                   // (p(p+1)/2 + lg) 8, p = 16 pb + lm
+#pragma unroll
+    for (int pb = 0; pb < QB; ++pb) {
+        const unsigned p = 16 * pb + lm;
+        ap[pb] = (((p * (p + 1)) >> 1) + lg) * 8;
+    }
     f64x4 G[QB];
 #pragma unroll
     for (int i = 0; i < QB; ++i) G[i] = f64x4{0.0, 0.0, 0.0, 0.0};
@@ -177,15 +209,12 @@ __global__ __launch_bounds__(256) void k_grad_mfma(const double* __restrict__ ht
         for (int e = 0; e < BP; ++e) {
             const bool ok = b + e < b1;
             const int64_t bb = ok ? b + e : b;
-            hr[e] = row_rsrc(ht + bb * pa, ok ? pa : 0);
-            gr[e] = row_rsrc(gam + bb * pm, ok ? pm : 0);
-            mult[e] = ok ? pair_mult(bb) : 0.0;
+            const bool one = bb == pm;                 // the one-electron term
+            hr[e] = row_rsrc(one ? h1 : ht + bb * pa, ok ? pa : 0);
+            gr[e] = row_rsrc(one ? g1 : gam + bb * pm, ok ? pm : 0);
+            mult[e] = ok ? (one ? 1.0 : pair_mult(bb)) : 0.0;
         }
         f64x4 U[BP][QB];
-#pragma unroll
-        for (int e = 0; e < BP; ++e)
-#pragma unroll
-            for (int i = 0; i < QB; ++i) U[e][i] = f64x4{0.0, 0.0, 0.0, 0.0};
 
         // stage 1: K step k covers nu = 4k + {0..3}; lane (lm, lg) supplies nu = 4k + lg.
         // A step past the last one loads only zeros or unused values (its offsets are past
@@ -194,7 +223,7 @@ __global__ __launch_bounds__(256) void k_grad_mfma(const double* __restrict__ ht
             const unsigned nu = 4 * k + lg;
             const unsigned ho = nu > mu ? packed_off(nu, mu) : packed_off(mu, nu);
 #pragma unroll
-            for (int e = 0; e < BP; ++e) hv[e] = bload(hr[e], ho, 0);
+            for (int e = 0; e < BP; ++e) hv[e] = bload(hr[e], ho, 0) * mult[e];
 #pragma unroll
             for (int qb = 0; qb < QB; ++qb) av[qb] = bload(crs, coff + k * 512, qb * (kCRows * 128));
         };
@@ -205,14 +234,33 @@ __global__ __launch_bounds__(256) void k_grad_mfma(const double* __restrict__ ht
                 for (int e = 0; e < BP; ++e)
                     U[e][qb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[qb], hv[e], U[e][qb], 0, 0, 0);
         };
+        // The first step starts U from a zero accumulator inside the MFMA, so U is an
+        // accumulator value from its first definition and stays in the accumulator
+        // registers through the loop (a zero initialised outside would be copied back
+        // from them on every iteration).  nk = 4 nmb >= 4.
         double a0[QB], a1[QB], h0[BP], h1[BP];
         ld1(0, a0, h0);
-        for (int k = 0; k < nk; k += 2) {
+        ld1(1, a1, h1);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+            for (int e = 0; e < BP; ++e)
+                U[e][qb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[qb], h0[e],
+                                                                f64x4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        ld1(2, a0, h0);
+        __builtin_amdgcn_sched_barrier(0);
+        mm1(a1, h1);
+        __builtin_amdgcn_sched_barrier(0);
+        for (int k = 2; k < nk; k += 2) {
             ld1(k + 1, a1, h1);
             __builtin_amdgcn_sched_barrier(0);
             mm1(a0, h0);
             __builtin_amdgcn_sched_barrier(0);
-            ld1(k + 2, a0, h0);
+            // the step after the last would read C rows np .. np + 3, past the padded
+            // copy's last block when np = kCRows: skipped (k is the same in every lane)
+            if (k + 2 < nk) ld1(k + 2, a0, h0);
             __builtin_amdgcn_sched_barrier(0);
             mm1(a1, h1);
             __builtin_amdgcn_sched_barrier(0);
@@ -223,20 +271,31 @@ __global__ __launch_bounds__(256) void k_grad_mfma(const double* __restrict__ ht
         constexpr int NG = BP * QB * 4;
         constexpr int RD = 3;
         double ring[RD][QB];
+        // Offsets of Gamma_b(q, p), q = 16 qb + 4 j + lg, p = 16 pb + lm, at pair(q, p):
+        //   pb < qb:  (q(q+1)/2 + lm) 8 + 128 pb            bq = first term, per group
+        //   pb > qb:  (p(p+1)/2 + lg) 8 + (16 qb + 4 j) 8   ap[pb] = first term, per lane
+        //   pb = qb:  the first form where q >= p, else the second.
+        // The constant parts go into the instruction's offset field, so the loads of a
+        // group cost no vector arithmetic beyond bq and the diagonal select.
         auto ld2 = [&](int g, double* gv) {
             const int e = g / (QB * 4), qb = (g >> 2) % QB, j = g & 3;
-            // The offsets depend only on the lane; an opaque copy of the lane index per
-            // group keeps the compiler from hoisting all QB * 4 * QB of them out of the
-            // loop over b (they would stay live in registers).
-            unsigned lgv = lg, lmv = lm;
-            asm volatile("" : "+v"(lgv), "+v"(lmv));
+            // bq and ap[pb] + cq depend only on the lane and the group; making the lane
+            // index and ap opaque per group keeps the compiler from hoisting all of them
+            // out of the loop over b (they would stay live in registers instead of going
+            // into the offset field).
+            unsigned lgv = lg;
+            asm volatile("" : "+v"(lgv));
+#pragma unroll
+            for (int pb = qb; pb < QB; ++pb) asm volatile("" : "+v"(ap[pb]));
             const unsigned q = 16 * qb + 4 * j + lgv;
+            const unsigned bq = (((q * (q + 1)) >> 1) + lm) * 8;
+            const unsigned cq = (16 * qb + 4 * j) * 8;
 #pragma unroll
             for (int pb = 0; pb < QB; ++pb) {
-                const unsigned p = 16 * pb + lmv;
-                const unsigned off = pb < qb ? packed_off(q, p)
-                                   : pb > qb ? packed_off(p, q)
-                                             : (q >= p ? packed_off(q, p) : packed_off(p, q));
+                unsigned off;
+                if (pb < qb) off = bq + 128 * pb;
+                else if (pb > qb) off = ap[pb] + cq;
+                else off = (4 * j + lgv >= (unsigned)lm) ? bq + 128 * pb : ap[pb] + cq;
                 gv[pb] = bload(gr[e], off, 0);
             }
         };
@@ -247,10 +306,10 @@ __global__ __launch_bounds__(256) void k_grad_mfma(const double* __restrict__ ht
             if (g + 2 < NG) ld2(g + 2, ring[(g + 2) % RD]);
             __builtin_amdgcn_sched_barrier(0);
             const int e = g / (QB * 4), qb = (g >> 2) % QB, j = g & 3;
-            const double a = U[e][qb][j] * mult[e];
 #pragma unroll
             for (int pb = 0; pb < QB; ++pb)
-                G[pb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ring[g % RD][pb], G[pb], 0, 0, 0);
+                G[pb] = __builtin_amdgcn_mfma_f64_16x16x4f64(U[e][qb][j], ring[g % RD][pb], G[pb], 0,
+                                                             0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
@@ -267,28 +326,32 @@ __global__ __launch_bounds__(256) void k_grad_mfma(const double* __restrict__ ht
     }
 }
 
-// grad(mu, p) = 2 (sum_chunks part[chunk][mu][p] + sum_q hc(mu, q) gamma(q, p)), written
-// column-major (ld n).  64 outputs per work-group; its four waves take the chunks
-// congruent to 0..3 mod 4, each in chunk order, and the four sums are added in a fixed
-// order.
+// grad(mu, p) = 2 sum_chunks part[chunk][mu][p], written column-major (ld n).  64 outputs
+// per work-group; wave s takes the chunks congruent to s mod 4, in four interleaved
+// running sums (chunks s, s + 4, s + 8, s + 12 mod 16), and all partial sums are added in
+// a fixed order.
 __global__ __launch_bounds__(256) void k_grad_reduce(const double* __restrict__ part, int64_t nch,
-                                                     const double* __restrict__ hc,
-                                                     const double* __restrict__ gamma, int n,
-                                                     int m, double* __restrict__ grad) {
+                                                     int n, int m, double* __restrict__ grad) {
     __shared__ double red[4][64];
     const int t = threadIdx.x & 63, s = threadIdx.x >> 6;
     const int64_t nm = (int64_t)n * m;
     const int64_t o = (int64_t)blockIdx.x * 64 + t;
-    double acc = 0.0;
-    if (o < nm)
-        for (int64_t c = s; c < nch; c += 4) acc += part[c * nm + o];
-    red[s][t] = acc;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (o < nm) {
+        int64_t c = s;
+        for (; c + 12 < nch; c += 16) {
+            a0 += part[c * nm + o];
+            a1 += part[(c + 4) * nm + o];
+            a2 += part[(c + 8) * nm + o];
+            a3 += part[(c + 12) * nm + o];
+        }
+        for (; c < nch; c += 4) a0 += part[c * nm + o];
+    }
+    red[s][t] = (a0 + a1) + (a2 + a3);
     __syncthreads();
     if (s != 0 || o >= nm) return;
     const int mu = (int)(o / m), p = (int)(o - (int64_t)mu * m);
-    double v = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-    for (int q = 0; q < m; ++q) v += hc[(int64_t)q * n + mu] * gamma[(int64_t)q * m + p];
-    grad[(int64_t)p * n + mu] = 2.0 * v;
+    grad[(int64_t)p * n + mu] = 2.0 * ((red[0][t] + red[1][t]) + (red[2][t] + red[3][t]));
 }
 
 // Work split of k_grad_mfma: chunks of MO pairs sized so that a launch has about 4096
@@ -306,12 +369,12 @@ GradPlan grad_plan(int n, int m) {
     g.nmb = (n + 15) / 16;
     g.qb = (m + 15) / 16;
     g.bp = g.qb <= kGradPairMaxQB ? 2 : 1;
-    const int64_t pm = npairs(m);
-    int64_t c = (pm * g.nmb + 4095) / 4096;
+    const int64_t nb = npairs(m) + 1;                  // MO pairs and the one-electron term
+    int64_t c = (nb * g.nmb + 4095) / 4096;
     c = c < 2 ? 2 : (c > 64 ? 64 : c);
     c = (c + g.bp - 1) / g.bp * g.bp;
     g.chunk = (int)c;
-    g.nch = (pm + c - 1) / c;
+    g.nch = (nb + c - 1) / c;
     g.nitems = g.nch * g.nmb;
     g.grid = xcd_grid((g.nitems + 3) / 4);
     return g;
@@ -319,27 +382,28 @@ GradPlan grad_plan(int n, int m) {
 
 template <int QB>
 hipError_t launch_grad_qb(const GradPlan& g, const double* ht, int64_t pa, const double* gam,
-                          int64_t pm, const double* cr, int n, int m, double* part,
-                          hipStream_t st) {
+                          int64_t pm, const double* cr, const double* h1, const double* g1, int n,
+                          int m, double* part, hipStream_t st) {
     if (g.bp == 2) {
         if constexpr (QB <= kGradPairMaxQB)
             hipLaunchKernelGGL((k_grad_mfma<QB, 2>), dim3((unsigned)g.grid), dim3(256), 0, st, ht,
-                               pa, gam, pm, cr, n, m, g.nmb, g.chunk, g.nitems, part);
+                               pa, gam, pm, cr, h1, g1, n, m, g.nmb, g.chunk, g.nitems, part);
         else
             return hipErrorInvalidValue;
     } else {
         hipLaunchKernelGGL((k_grad_mfma<QB, 1>), dim3((unsigned)g.grid), dim3(256), 0, st, ht, pa,
-                           gam, pm, cr, n, m, g.nmb, g.chunk, g.nitems, part);
+                           gam, pm, cr, h1, g1, n, m, g.nmb, g.chunk, g.nitems, part);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_grad(const GradPlan& g, const double* ht, int64_t pa, const double* gam,
-                       int64_t pm, const double* cr, int n, int m, double* part, hipStream_t st) {
+                       int64_t pm, const double* cr, const double* h1, const double* g1, int n,
+                       int m, double* part, hipStream_t st) {
     if (g.grid > 0x7fffffff || n > kCRows || npairs(n) * 8 > 0x7fffffff) return hipErrorInvalidValue;
     switch (g.qb) {
 #define CIPSI_GRAD_CASE(Q) \
-    case Q: return launch_grad_qb<Q>(g, ht, pa, gam, pm, cr, n, m, part, st);
+    case Q: return launch_grad_qb<Q>(g, ht, pa, gam, pm, cr, h1, g1, n, m, part, st);
         CIPSI_GRAD_CASE(1) CIPSI_GRAD_CASE(2) CIPSI_GRAD_CASE(3) CIPSI_GRAD_CASE(4)
         CIPSI_GRAD_CASE(5) CIPSI_GRAD_CASE(6) CIPSI_GRAD_CASE(7) CIPSI_GRAD_CASE(8)
         CIPSI_GRAD_CASE(9) CIPSI_GRAD_CASE(10) CIPSI_GRAD_CASE(11) CIPSI_GRAD_CASE(12)
@@ -361,14 +425,15 @@ static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, co
     if ((rc = ctx_alloc(ctx, ctx->adj_g, sizeof(double) * nout))) return rc;
     const GradPlan gp = grad_plan(n, m);
     if ((rc = ctx_alloc(ctx, ctx->adj_small,
-                        sizeof(double) * (pm + (1 + (size_t)gp.nch) * n * m + 4096))))
+                        sizeof(double) * (3 * pm + (size_t)gp.nch * n * m + 4096))))
         return rc;
     double* vsq = (double*)ctx->vsq.ptr;
     double* h = (double*)ctx->h.ptr;
     double* g = (double*)ctx->adj_g.ptr;
     double* hmo = (double*)ctx->adj_small.ptr;
-    double* hc = hmo + pm;
-    double* part = hc + (size_t)n * m;                // energy partials (4096), then grad
+    double* mult = hmo + pm;                           // m(b) per MO pair
+    double* gs = mult + pm;                            // symmetrised gamma, packed
+    double* part = gs + pm;                            // energy partials (4096), then grad
     double* gpart = part + 4096;                       // partials (nch x n x m)
     double* cr;
     if ((rc = run_pad_c(ctx, n, m, d_c, ldc, &cr))) return rc;
@@ -404,8 +469,11 @@ static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, co
     const int nblk = 1024;
     {
         StageTimer tm(ctx, "energy");
+        hipLaunchKernelGGL(k_pair_tables, dim3((unsigned)((pm + 255) / 256)), dim3(256), 0,
+                           ctx->stream, d_gamma, m, pm, mult, gs);
+        const int64_t len = ((nout + nblk - 1) / nblk + RB - 1) / RB * RB;
         hipLaunchKernelGGL(k_energy2_partial, dim3(nblk), dim3(RB), 0, ctx->stream, g, d_Gamma,
-                           nout, part);
+                           mult, nout, len, part);
         hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(RB), 0, ctx->stream, part, nblk, hmo,
                            d_gamma, m, d_energy);
     }
@@ -413,24 +481,24 @@ static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, co
     if (!d_grad) return CIPSI_OK;
 
     // gradient: Gamma expanded to the full P_M x P_M (row b = Gamma_b packed), then
-    // 2 sum_b m(b) H_b C Gamma_b from the rows of H^T still in vsq, and the reduction
+    // 2 sum_b m(b) H_b C Gamma_b + 2 h C gs from the rows of H^T still in vsq, and the
+    // reduction
     if ((rc = ctx_alloc(ctx, ctx->adj_gh, sizeof(double) * pm * pm))) return rc;
     double* gfull = (double*)ctx->adj_gh.ptr;
     {
         StageTimer tm(ctx, "grad_gamma");
         if (launch_expand_packed(d_Gamma, pm, gfull, ctx->stream) != hipSuccess)
             return CIPSI_ERR_HIP;
-        hipLaunchKernelGGL(k_hc, dim3((n * m + 255) / 256), dim3(256), 0, ctx->stream, d_hp, d_c,
-                           ldc, n, m, hc);
     }
     {
         StageTimer tm(ctx, "grad_mfma");
-        if (launch_grad(gp, vsq, pa, gfull, pm, cr, n, m, gpart, ctx->stream) != hipSuccess)
+        if (launch_grad(gp, vsq, pa, gfull, pm, cr, d_hp, gs, n, m, gpart, ctx->stream) !=
+            hipSuccess)
             return CIPSI_ERR_HIP;
     }
     StageTimer tm(ctx, "grad_reduce");
     hipLaunchKernelGGL(k_grad_reduce, dim3((unsigned)((n * m + 63) / 64)), dim3(256), 0,
-                       ctx->stream, gpart, gp.nch, hc, d_gamma, n, m, d_grad);
+                       ctx->stream, gpart, gp.nch, n, m, d_grad);
     return hipGetLastError() == hipSuccess ? CIPSI_OK : CIPSI_ERR_HIP;
 }
 
