@@ -286,7 +286,8 @@ __global__ __launch_bounds__(256) void k_grad_mfma(const double* __restrict__ ht
             unsigned lgv = lg;
             asm volatile("" : "+v"(lgv));
 #pragma unroll
-            for (int pb = qb; pb < QB; ++pb) asm volatile("" : "+v"(ap[pb]));
+            for (int pb = 0; pb < QB; ++pb)
+                if (pb >= qb) asm volatile("" : "+v"(ap[pb]));
             const unsigned q = 16 * qb + 4 * j + lgv;
             const unsigned bq = (((q * (q + 1)) >> 1) + lm) * 8;
             const unsigned cq = (16 * qb + 4 * j) * 8;

@@ -290,6 +290,12 @@ int cipsi_ijkl_destroy(cipsi_ijkl_ctx* ctx) {
     cipsi_ijkl_reset_timing(ctx);
     cipsi_ijkl_release_workspace(ctx);
     pin_free(ctx);
+    for (hipEvent_t& e : ctx->seg_ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    if (ctx->front_ev) (void)hipEventDestroy(ctx->front_ev);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return CIPSI_OK;
@@ -314,6 +320,35 @@ int cipsi_ijkl_synchronize(cipsi_ijkl_ctx* ctx) {
     return CIPSI_OK;
 }
 
+}  // extern "C"
+
+// A transform up to half 2: the padded C, the expanded pair matrix, half 1 and the
+// transpose.  On return *ht is H^T (P_M rows of P_A) and *cr the padded C.
+static int transform_front(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* d_ao,
+                           const double* d_c, int ldc, int precision, int n_slices,
+                           const double** ht, double** cr, bool expanded = false) {
+    const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
+    int rc;
+    if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->h, sizeof(double) * pa * pm))) return rc;
+    if ((rc = run_pad_c(ctx, n_ao, n_mo, d_c, ldc, cr))) return rc;
+    double* vsq = (double*)ctx->vsq.ptr;
+    double* h = (double*)ctx->h.ptr;
+    if (!expanded) {
+        StageTimer tm(ctx, "expand");
+        HIP_TRY(ctx, launch_expand_packed(d_ao, pa, vsq, ctx->stream));
+    }
+    if ((rc = run_half(ctx, 1, n_ao, n_mo, vsq, *cr, h, precision, n_slices))) return rc;
+    {
+        StageTimer tm(ctx, "transpose");
+        HIP_TRY(ctx, launch_transpose(h, pa, pm, vsq, ctx->stream));   // H^T reuses vsq
+    }
+    *ht = vsq;
+    return CIPSI_OK;
+}
+
+extern "C" {
+
 int64_t cipsi_ijkl_workspace_bytes(int n_ao, int n_mo, int precision, int with_labels) {
     if (check_dims(n_ao, n_mo, n_ao)) return -1;
     const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
@@ -331,24 +366,11 @@ int cipsi_ijkl_transform_packed_dev(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo,
     if ((rc = check_dims(n_ao, n_mo, ldc))) return rc;
     if ((rc = check_prec(precision, n_slices))) return rc;
     if (!d_ao || !d_c || !d_mo) return CIPSI_ERR_ARG;
-    const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
-    if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
-    if ((rc = ctx_alloc(ctx, ctx->h, sizeof(double) * pa * pm))) return rc;
+    const double* ht;
     double* cr;
-    if ((rc = run_pad_c(ctx, n_ao, n_mo, d_c, ldc, &cr))) return rc;
-    double* vsq = (double*)ctx->vsq.ptr;
-    double* h = (double*)ctx->h.ptr;
-    {
-        StageTimer tm(ctx, "expand");
-        HIP_TRY(ctx, launch_expand_packed(d_ao, pa, vsq, ctx->stream));
-    }
-    if ((rc = run_half(ctx, 1, n_ao, n_mo, vsq, cr, h, precision, n_slices))) return rc;
-    {
-        StageTimer tm(ctx, "transpose");
-        HIP_TRY(ctx, launch_transpose(h, pa, pm, vsq, ctx->stream));   // H^T reuses vsq
-    }
-    if ((rc = run_half(ctx, 2, n_ao, n_mo, vsq, cr, d_mo, precision, n_slices))) return rc;
-    return CIPSI_OK;
+    if ((rc = transform_front(ctx, n_ao, n_mo, d_ao, d_c, ldc, precision, n_slices, &ht, &cr)))
+        return rc;
+    return run_half(ctx, 2, n_ao, n_mo, ht, cr, d_mo, precision, n_slices);
 }
 
 int cipsi_ijkl_unpack_labels_dev(cipsi_ijkl_ctx* ctx, int n_ao, const double* d_xx,
@@ -433,14 +455,26 @@ int cipsi_ijkl_half2_dev(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* 
 // slots: host threads copy a chunk between the caller's (pageable) memory and a slot
 // while the DMA engine moves the previous slot, so the two overlap.  A slot is reused
 // only after the event of its last DMA has completed.
-static constexpr int kCopyThreads = 8;
+// Up to 16 copy threads: the caller's buffer may be fresh memory, whose first touch
+// (the kernel zeroing each page) halves a thread's copy rate, and the copy into it must
+// keep up with the DMA engine.
+static constexpr int kCopyThreads = 16;
+
+static int copy_threads() {
+    static const int n = [] {
+        const unsigned hw = std::thread::hardware_concurrency();
+        return hw == 0 ? 8 : (int)std::min<unsigned>(hw, kCopyThreads);
+    }();
+    return n;
+}
 
 static void parallel_memcpy(void* dst, const void* src, size_t bytes) {
-    if (bytes < (size_t(4) << 20)) {
+    const int nthr = copy_threads();
+    if (bytes < (size_t(4) << 20) || nthr < 2) {
         std::memcpy(dst, src, bytes);
         return;
     }
-    const size_t part = ((bytes / kCopyThreads) + 63) & ~size_t(63);
+    const size_t part = ((bytes / nthr) + 63) & ~size_t(63);
     std::thread th[kCopyThreads];
     int used = 0;
     for (size_t off = 0; off < bytes; off += part) {
@@ -454,6 +488,53 @@ static void parallel_memcpy(void* dst, const void* src, size_t bytes) {
     }
     for (int t = 0; t < used; ++t) th[t].join();
 }
+
+// Touches every page of the caller's output buffer from background threads.  Fresh
+// memory faults on its first touch (the kernel zeroes each page), which halves the rate
+// of the copy into it; touching it while the host waits for the upload and the
+// transform takes that work off the copy.  join() must come before anything else writes
+// the buffer; the destructor joins on every path.
+static bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
+    const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+    return x < y + nb && y < x + na;
+}
+
+class PageToucher {
+  public:
+    // No touching when the output overlaps an input the call still has to read.
+    PageToucher(void* p, size_t bytes, const void* in1, size_t n1, const void* in2 = nullptr,
+                size_t n2 = 0, const void* in3 = nullptr, size_t n3 = 0) {
+        if (overlaps(p, bytes, in1, n1) || (in2 && overlaps(p, bytes, in2, n2)) ||
+            (in3 && overlaps(p, bytes, in3, n3)))
+            return;
+        constexpr size_t kPage = 4096;
+        const size_t part = ((bytes / kThreads) + kPage - 1) & ~(kPage - 1);
+        for (size_t off = 0; off < bytes && used_ < kThreads; off += part) {
+            const size_t end = std::min(bytes, off + part);
+            try {
+                th_[used_] = std::thread([=] {
+                    volatile char* q = (volatile char*)p;
+                    for (size_t o = off; o < end; o += kPage) q[o] = 0;
+                });
+                ++used_;
+            } catch (...) {
+                break;                 // no thread: the copy faults the pages itself
+            }
+        }
+    }
+    void join() {
+        for (int t = 0; t < used_; ++t) th_[t].join();
+        used_ = 0;
+    }
+    ~PageToucher() { join(); }
+    PageToucher(const PageToucher&) = delete;
+    PageToucher& operator=(const PageToucher&) = delete;
+
+  private:
+    static constexpr int kThreads = 8;
+    std::thread th_[kThreads];
+    int used_ = 0;
+};
 
 static int pin_ready(cipsi_ijkl_ctx* ctx) {
     for (int s = 0; s < cipsi_ijkl_ctx::kPinSlots; ++s) {
@@ -496,7 +577,8 @@ extern "C" {
 
 // Ordered after the work already queued on the context stream; returns when the caller's
 // buffer holds the data.
-static int copy_d2h(cipsi_ijkl_ctx* ctx, void* h, const void* d, size_t bytes) {
+static int copy_d2h(cipsi_ijkl_ctx* ctx, void* h, const void* d, size_t bytes,
+                    PageToucher* touch = nullptr) {
     int rc = pin_ready(ctx);
     if (rc) return rc;
     constexpr int S = cipsi_ijkl_ctx::kPinSlots;
@@ -515,6 +597,7 @@ static int copy_d2h(cipsi_ijkl_ctx* ctx, void* h, const void* d, size_t bytes) {
     };
     for (int64_t k = 0; k < nch && k < S; ++k)
         if ((rc = issue(k))) return rc;
+    if (touch) touch->join();
     for (int64_t k = 0; k < nch; ++k) {
         const int s = (int)(k % S);
         const size_t off = (size_t)k * chunk, len = std::min(chunk, bytes - off);
@@ -523,6 +606,167 @@ static int copy_d2h(cipsi_ijkl_ctx* ctx, void* h, const void* d, size_t bytes) {
         if (k + S < nch && (rc = issue(k + S))) return rc;
     }
     return CIPSI_OK;
+}
+
+// FP64 half 2 of a host entry point with the copy of its output to the caller running
+// behind it.  The MO stream is ordered by MO pair row kl = pair0(l, k), so the rows with
+// l < L are the prefix of npairs(npairs(L)) elements, and half 2's work items are ordered
+// by l too.  Half 2 is launched in up to kOutSegments ranges of l, each holding an equal
+// share of the output and each followed by an event; a second stream copies the stream
+// chunk by chunk through the pinned slots, every chunk after the event of the segment
+// that completes it.  The PCIe copy (about 60 ms at N_MO = 240) then starts a few
+// milliseconds into half 2 instead of after it.
+static int half2_stream_out(cipsi_ijkl_ctx* ctx, int n, int m, const double* ht,
+                            const double* cr, double* d_mo, double* mo, PageToucher* touch) {
+    const int var = ctx_half_variant(ctx, 2, n, m, CIPSI_PREC_FP64);
+    if (!variant_supports(var, row_class(round16(n)))) return CIPSI_ERR_UNSUPPORTED;
+    const int lt = variant_columns(var);
+    const int64_t* ts;
+    int64_t total;
+    int rc = ctx_tile_table(ctx, m, lt, &ts, &total);
+    if (rc) return rc;
+    if ((rc = pin_ready(ctx))) return rc;
+    if (!ctx->copy_stream)
+        HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    for (hipEvent_t& e : ctx->seg_ev)
+        if (!e) HIP_TRY(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // work items before row l (the device tile table, on the host)
+    std::vector<int64_t> items(m + 1, 0);
+    for (int l = 0; l < m; ++l) items[l + 1] = items[l] + (int64_t)(l + 1) * ((l + lt) / lt);
+    auto out_before = [](int64_t l) { return npairs(npairs(l)); };   // elements of rows l' < l
+    // segment s ends where the output reaches (s / K)^2 of the whole: the first segments
+    // are small, so the copy starts early, and each later one adds a little more
+    constexpr int K = cipsi_ijkl_ctx::kOutSegments;
+    const int64_t nout = out_before(m);
+    int bound[K + 1];
+    int nseg = 0;
+    bound[0] = 0;
+    for (int s = 1; s <= K; ++s) {
+        const int64_t goal = (int64_t)((double)nout * s * s / (K * K));
+        int l = bound[nseg];
+        while (l < m && out_before(l) < goal) ++l;
+        if (l > bound[nseg]) bound[++nseg] = l;
+    }
+    const int64_t pa = npairs(n), pm = npairs(m);
+    HalfArgs a{};
+    a.src = ht;
+    a.src_ld = pa;
+    a.cr = cr;
+    a.ldm = round16(m);
+    a.dst = d_mo;
+    a.n = n;
+    a.m = m;
+    a.np = round16(n);
+    a.mp = round16(m);
+    a.half = 2;
+    a.nb = pm;
+    a.tile_start = ts;
+    {
+        StageTimer tm(ctx, "half2");
+        for (int s = 0; s < nseg; ++s) {
+            a.item_begin = items[bound[s]];
+            a.ntile_total = items[bound[s + 1]];
+            HIP_TRY(ctx, launch_half_fp64(a, var, ctx->stream));
+            HIP_TRY(ctx, hipEventRecord(ctx->seg_ev[s], ctx->stream));
+        }
+    }
+    constexpr int S = cipsi_ijkl_ctx::kPinSlots;
+    const size_t chunk = cipsi_ijkl_ctx::kPinBytes;
+    const size_t bytes = sizeof(double) * (size_t)nout;
+    const int64_t nch = (int64_t)((bytes + chunk - 1) / chunk);
+    int waited = -1;                               // last segment the copy stream waits for
+    auto issue = [&](int64_t k) -> int {
+        const int s = (int)(k % S);
+        const size_t off = (size_t)k * chunk, len = std::min(chunk, bytes - off);
+        int r = pin_wait(ctx, s);
+        if (r) return r;
+        const int64_t end = (int64_t)((off + len) / sizeof(double));
+        int g = waited < 0 ? 0 : waited;
+        while (out_before(bound[g + 1]) < end) ++g;
+        if (g > waited) {
+            HIP_TRY(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->seg_ev[g], 0));
+            waited = g;
+        }
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->pin[s], (const char*)d_mo + off, len,
+                                    hipMemcpyDeviceToHost, ctx->copy_stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->pin_ev[s], ctx->copy_stream));
+        ctx->pin_busy[s] = true;
+        return CIPSI_OK;
+    };
+    for (int64_t k = 0; k < nch && k < S; ++k)
+        if ((rc = issue(k))) return rc;
+    touch->join();
+    for (int64_t k = 0; k < nch; ++k) {
+        const int s = (int)(k % S);
+        const size_t off = (size_t)k * chunk, len = std::min(chunk, bytes - off);
+        if ((rc = pin_wait(ctx, s))) return rc;
+        parallel_memcpy((char*)mo + off, ctx->pin[s], len);
+        if (k + S < nch && (rc = issue(k + S))) return rc;
+    }
+    return CIPSI_OK;
+}
+
+// The upload of the packed AO block for the FP64 host entry point, with the expand run
+// behind it: every chunk goes through the pinned slots on the context stream, and as soon
+// as a chunk completes the packed rows of some expand tile rows, those tile rows are
+// expanded on the copy stream after the chunk's event.  front_ev marks the last of them;
+// the caller makes the context stream wait for it before half 1.
+static int upload_expand(cipsi_ijkl_ctx* ctx, double* d_in, const double* ao, int64_t pa,
+                         double* vsq) {
+    int rc = pin_ready(ctx);
+    if (rc) return rc;
+    if (!ctx->copy_stream)
+        HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    if (!ctx->front_ev) HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->front_ev, hipEventDisableTiming));
+    const size_t chunk = cipsi_ijkl_ctx::kPinBytes;
+    const size_t bytes = sizeof(double) * (size_t)npairs(pa);
+    const int64_t ntr = (pa + kExpandTile - 1) / kExpandTile;
+    int64_t done = 0;                                  // tile rows expanded so far
+    int s = 0;
+    for (size_t off = 0; off < bytes; off += chunk, s = (s + 1) % cipsi_ijkl_ctx::kPinSlots) {
+        const size_t len = std::min(chunk, bytes - off);
+        if ((rc = pin_wait(ctx, s))) return rc;
+        parallel_memcpy(ctx->pin[s], (const char*)ao + off, len);
+        HIP_TRY(ctx, hipMemcpyAsync((char*)d_in + off, ctx->pin[s], len, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->pin_ev[s], ctx->stream));
+        ctx->pin_busy[s] = true;
+        // packed rows a with npairs(a + 1) <= elements uploaded are complete
+        const int64_t have = (int64_t)((off + len) / sizeof(double));
+        int64_t rows = (int64_t)((std::sqrt(8.0 * (double)have + 1.0) - 1.0) * 0.5);
+        while (rows > 0 && npairs(rows) > have) --rows;
+        while (npairs(rows + 1) <= have) ++rows;
+        const int64_t tr = off + len == bytes ? ntr : std::min(ntr, rows / kExpandTile);
+        if (tr > done) {
+            HIP_TRY(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->pin_ev[s], 0));
+            HIP_TRY(ctx, launch_expand_rows(d_in, pa, vsq, done, tr, ctx->copy_stream));
+            done = tr;
+        }
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->front_ev, ctx->copy_stream));
+    return CIPSI_OK;
+}
+
+// The device part of a host entry point from the resident packed block on: FP64 streams
+// its output to the caller during half 2, the split path copies it after the transform.
+// expanded: the pair matrix is already in vsq (upload_expand).
+static int transform_to_host(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* d_ao,
+                             const double* d_c, int ldc, int precision, int n_slices,
+                             double* mo, bool expanded, PageToucher* touch) {
+    double* d_mo = (double*)ctx->io_out.ptr;
+    int rc;
+    if (precision == CIPSI_PREC_FP64) {
+        const double* ht;
+        double* cr;
+        if ((rc = transform_front(ctx, n_ao, n_mo, d_ao, d_c, ldc, precision, n_slices, &ht, &cr,
+                                  expanded)))
+            return rc;
+        return half2_stream_out(ctx, n_ao, n_mo, ht, cr, d_mo, mo, touch);
+    }
+    if ((rc = cipsi_ijkl_transform_packed_dev(ctx, n_ao, n_mo, d_ao, d_c, ldc, d_mo, precision,
+                                              n_slices)))
+        return rc;
+    return copy_d2h(ctx, mo, d_mo, sizeof(double) * npairs(npairs(n_mo)), touch);
 }
 
 static void pin_free(cipsi_ijkl_ctx* ctx) {
@@ -590,13 +834,28 @@ int cipsi_ijkl_transform_packed(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
     if ((rc = ctx_alloc(ctx, ctx->io_out, sizeof(double) * nout))) return rc;
     double* d_c;
     if ((rc = upload_c(ctx, n_ao, n_mo, c, ldc, &d_c))) return rc;
-    if ((rc = ctx_copy_h2d(ctx, ctx->io_in.ptr, ao, sizeof(double) * nin))) return rc;
-    double vmax = 0.0;
-    if ((rc = scan_uploaded(ctx, (const double*)ctx->io_in.ptr, nin, nullptr, &vmax))) return rc;
-    if ((rc = cipsi_ijkl_transform_packed_dev(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr, d_c,
-                                              ldc, (double*)ctx->io_out.ptr, precision, n_slices)))
+    PageToucher touch(mo, sizeof(double) * nout, ao, sizeof(double) * nin, c,
+                      sizeof(double) * (size_t)ldc * n_mo);
+    // FP64: the expand runs during the upload (the scan below still rejects a non-finite
+    // block before half 1 starts)
+    const bool fp64 = precision == CIPSI_PREC_FP64;
+    if (fp64) {
+        const int64_t pa = npairs(n_ao);
+        if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
+        if ((rc = upload_expand(ctx, (double*)ctx->io_in.ptr, ao, pa, (double*)ctx->vsq.ptr)))
+            return rc;
+    } else if ((rc = ctx_copy_h2d(ctx, ctx->io_in.ptr, ao, sizeof(double) * nin))) {
         return rc;
-    if ((rc = copy_d2h(ctx, mo, ctx->io_out.ptr, sizeof(double) * nout))) return rc;
+    }
+    double vmax = 0.0;
+    if ((rc = scan_uploaded(ctx, (const double*)ctx->io_in.ptr, nin, nullptr, &vmax))) {
+        if (fp64) (void)hipStreamSynchronize(ctx->copy_stream);
+        return rc;
+    }
+    if (fp64) HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->front_ev, 0));
+    if ((rc = transform_to_host(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr, d_c, ldc,
+                                precision, n_slices, mo, fp64, &touch)))
+        return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (err_est) *err_est = err_estimate(n_ao, precision, n_slices, vmax);
     return CIPSI_OK;
@@ -618,6 +877,8 @@ int cipsi_ijkl_transform_labels(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
     if ((rc = ctx_alloc(ctx, ctx->io_out, sizeof(double) * nout))) return rc;
     double* d_c;
     if ((rc = upload_c(ctx, n_ao, n_mo, c, ldc, &d_c))) return rc;
+    PageToucher touch(mo, sizeof(double) * nout, xx, sizeof(double) * n_entries, labels,
+                      sizeof(int32_t) * n_entries, c, sizeof(double) * (size_t)ldc * n_mo);
     if (n_entries > 0) {
         if ((rc = ctx_copy_h2d(ctx, ctx->io_in.ptr, xx, sizeof(double) * n_entries))) return rc;
         if ((rc = ctx_copy_h2d(ctx, ctx->io_in2.ptr, labels, sizeof(int32_t) * n_entries))) return rc;
@@ -627,11 +888,14 @@ int cipsi_ijkl_transform_labels(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
                             (const int32_t*)ctx->io_in2.ptr, &vmax)))
         return rc;
     vmax *= 8.0;
-    if ((rc = cipsi_ijkl_transform_labels_dev(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr,
-                                              (const int32_t*)ctx->io_in2.ptr, n_entries, d_c, ldc,
-                                              (double*)ctx->io_out.ptr, precision, n_slices)))
+    if ((rc = ctx_alloc(ctx, ctx->packed, sizeof(double) * npairs(npairs(n_ao))))) return rc;
+    if ((rc = cipsi_ijkl_unpack_labels_dev(ctx, n_ao, (const double*)ctx->io_in.ptr,
+                                           (const int32_t*)ctx->io_in2.ptr, n_entries,
+                                           (double*)ctx->packed.ptr)))
         return rc;
-    if ((rc = copy_d2h(ctx, mo, ctx->io_out.ptr, sizeof(double) * nout))) return rc;
+    if ((rc = transform_to_host(ctx, n_ao, n_mo, (const double*)ctx->packed.ptr, d_c, ldc,
+                                precision, n_slices, mo, false, &touch)))
+        return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (err_est) *err_est = err_estimate(n_ao, precision, n_slices, vmax);
     return CIPSI_OK;

@@ -59,7 +59,8 @@ struct HalfArgs {
     int64_t nb;            // number of matrices (P_A or P_M)
     int64_t b_begin;       // first matrix index handled by this launch
     int64_t dst_ld;        // half 1: P_M
-    int64_t ntile_total;   // number of (b, column tile) work items
+    int64_t ntile_total;   // end of the (b, column tile) work items of this launch
+    int64_t item_begin;    // first work item of this launch (0: the whole half)
     const int64_t* tile_start;  // half 2: prefix table over l (m + 1 entries), device
 };
 
@@ -81,6 +82,11 @@ hipError_t launch_unpack_labels(const double* xx, const int32_t* labels, int64_t
                                 int n_ao, double* ao_packed, uint32_t* owner,
                                 unsigned long long* sc, int* err_flag, hipStream_t st);
 hipError_t launch_expand_packed(const double* ao_packed, int64_t pa, double* vsq, hipStream_t st);
+// Rows of 64 x 64 expand tiles: tile row i covers pair-matrix rows 64 i .. 64 i + 63 and
+// reads only packed rows 64 i .. 64 i + 63.
+constexpr int kExpandTile = 64;
+hipError_t launch_expand_rows(const double* ao_packed, int64_t pa, double* vsq, int64_t i0,
+                              int64_t i1, hipStream_t st);
 // Input scan for the host entry points: s[0] = bits of max |x|, s[1] = any non-finite,
 // over p[0 .. end) with end = the first zero label's index (labels given) or n.  p must
 // be 16-byte aligned; s holds three 64-bit words.

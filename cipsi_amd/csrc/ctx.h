@@ -38,6 +38,12 @@ struct cipsi_ijkl_ctx {
     hipEvent_t pin_ev[kPinSlots] = {};
     bool pin_busy[kPinSlots] = {};
     DevBuf io_in, io_in2, io_out, io_c;   // staging for host-pointer entry points
+    // host entry points: the stream that copies finished parts of the MO stream to the
+    // caller while half 2 still runs, and one event per half-2 segment (first use)
+    static constexpr int kOutSegments = 16;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t seg_ev[kOutSegments] = {};
+    hipEvent_t front_ev = nullptr;   // the last expand tile row run during the upload
     // Half-2 work tables, one per (N_MO, columns per work-group), written once and kept
     // until release_workspace: a graph captured from an earlier call keeps its pointer.
     struct TileTable { DevBuf buf; int64_t total = 0; };

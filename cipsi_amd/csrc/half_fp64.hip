@@ -272,7 +272,7 @@ int64_t xcd_grid(int64_t items) {
 
 template <int NW, int NRB, int KC, int MINB>
 __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
-    const int64_t item = xcd_item(blockIdx.x);
+    const int64_t item = A.item_begin + xcd_item(blockIdx.x);
     if (item >= A.ntile_total) return;
     half_tile<NW, NRB, KC>(A, item);
 }
@@ -315,7 +315,9 @@ static hipError_t launch_v(const HalfArgs& a, hipStream_t st) {
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    const int64_t g = xcd_grid(a.ntile_total);
+    if (a.item_begin < 0 || a.item_begin > a.ntile_total) return hipErrorInvalidValue;
+    const int64_t g = xcd_grid(a.ntile_total - a.item_begin);
+    if (g == 0) return hipSuccess;
     if (g > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_half_fp64<NW, NRB, KC, MINB>), dim3((unsigned)g), dim3(K::NT), shmem, st, a);
     return hipGetLastError();

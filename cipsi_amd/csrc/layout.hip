@@ -364,7 +364,7 @@ hipError_t launch_scan_input(const double* p, int64_t n, const int32_t* labels,
 // wants row pq of the symmetric pair matrix contiguous.  64 x 64 tiles over the
 // lower triangle: read the packed tile (rows contiguous), write it straight and
 // transposed through LDS, both coalesced.
-constexpr int XT = 64;
+constexpr int XT = kExpandTile;
 
 __device__ __forceinline__ void expand_tile(const double* __restrict__ v, int64_t pa,
                                             double* __restrict__ vsq, int64_t t) {
@@ -389,15 +389,24 @@ __device__ __forceinline__ void expand_tile(const double* __restrict__ v, int64_
 }
 
 __global__ __launch_bounds__(256) void k_expand_packed(const double* __restrict__ v, int64_t pa,
-                                                       double* __restrict__ vsq) {
-    expand_tile(v, pa, vsq, blockIdx.x);
+                                                       double* __restrict__ vsq, int64_t t0) {
+    expand_tile(v, pa, vsq, t0 + blockIdx.x);
 }
 
 int64_t expand_tiles(int64_t pa) { return npairs((pa + XT - 1) / XT); }
 
 hipError_t launch_expand_packed(const double* ao_packed, int64_t pa, double* vsq, hipStream_t st) {
-    const int64_t ntiles = expand_tiles(pa);
-    hipLaunchKernelGGL(k_expand_packed, dim3((unsigned)ntiles), dim3(256), 0, st, ao_packed, pa, vsq);
+    return launch_expand_rows(ao_packed, pa, vsq, 0, (pa + XT - 1) / XT, st);
+}
+
+// Tile rows [i0, i1): the tiles of rows 64 i0 .. 64 i1 - 1 of the pair matrix, which read
+// only those rows of the packed block (a diagonal tile reads its own rows transposed).
+hipError_t launch_expand_rows(const double* ao_packed, int64_t pa, double* vsq, int64_t i0,
+                              int64_t i1, hipStream_t st) {
+    const int64_t t0 = npairs(i0), nt = npairs(i1) - t0;
+    if (nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_expand_packed, dim3((unsigned)nt), dim3(256), 0, st, ao_packed, pa, vsq,
+                       t0);
     return hipGetLastError();
 }
 

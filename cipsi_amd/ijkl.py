@@ -284,8 +284,9 @@ class Context:
         return c, c.shape[0]
 
     # --------------------------------------------------------------- operators
-    def transform(self, ao_packed, c, precision="fp64", n_slices=0, return_err=False):
-        """Packed AO block -> packed MO stream (the whole C1 itijkl path)."""
+    def transform(self, ao_packed, c, precision="fp64", n_slices=0, return_err=False, out=None):
+        """Packed AO block -> packed MO stream (the whole C1 itijkl path).  Host arrays
+        may pass `out`, a float64 array of P_M(P_M+1)/2 elements reused across calls."""
         prec = _PREC[precision]
         cm, ldc = self._cmat(c)
         n, m = c.shape
@@ -303,7 +304,11 @@ class Context:
         ao = np.ascontiguousarray(ao_packed, dtype=np.float64)
         if ao.size != n_pairs(n_pairs(n)):
             raise ValueError("ao_packed has the wrong length")
-        out = np.empty(nout)
+        if out is None:
+            out = np.empty(nout)
+        elif (not isinstance(out, np.ndarray) or out.dtype != np.float64 or out.size != nout
+              or not out.flags.c_contiguous):
+            raise ValueError("out must be a contiguous float64 array of P_M(P_M+1)/2 elements")
         err = ctypes.c_double(0.0)
         self.lib.cipsi_ijkl_reset_stream(self.handle)
         self._check(self.lib.cipsi_ijkl_transform_packed(
@@ -398,8 +403,9 @@ class Context:
         self._batch_keep = keep    # column-major copies of C stay alive until the next call
         return mos
 
-    def itijkl(self, xx, ix, c, precision="fp64", n_slices=0, return_err=False):
-        """pshf label stream (values, 32-bit labels) -> packed MO stream."""
+    def itijkl(self, xx, ix, c, precision="fp64", n_slices=0, return_err=False, out=None):
+        """pshf label stream (values, 32-bit labels) -> packed MO stream.  Host arrays may
+        pass `out`, a float64 array of P_M(P_M+1)/2 elements reused across calls."""
         prec = _PREC[precision]
         cm, ldc = self._cmat(c)
         n, m = c.shape
@@ -414,7 +420,11 @@ class Context:
             return out
         xx = np.ascontiguousarray(xx, dtype=np.float64)
         ix = np.ascontiguousarray(ix, dtype=np.int32)
-        out = np.empty(nout)
+        if out is None:
+            out = np.empty(nout)
+        elif (not isinstance(out, np.ndarray) or out.dtype != np.float64 or out.size != nout
+              or not out.flags.c_contiguous):
+            raise ValueError("out must be a contiguous float64 array of P_M(P_M+1)/2 elements")
         err = ctypes.c_double(0.0)
         self.lib.cipsi_ijkl_reset_stream(self.handle)
         self._check(self.lib.cipsi_ijkl_transform_labels(

@@ -76,7 +76,9 @@ int cipsi_ijkl_release_workspace(cipsi_ijkl_ctx *ctx);
 /* Whole transform of a packed AO block: replaces `call itijkl(bijkl,bijkl,ydp)`
    (ijkl.f:318) for C1 with YDP=T, i.e. tpqkld (itijkl.f:214-220) followed by tijkld
    (itijkl.f:253-285), without the unit-40 scratch file.  err_est (may be NULL)
-   receives an a-priori bound on max |error| of the output. */
+   receives an a-priori bound on max |error| of the output.  The copy of mo to the
+   caller runs while half 2 still computes (FP64); after an error return the contents
+   of mo are unspecified. */
 int cipsi_ijkl_transform_packed(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo,
                                 const double *ao_packed, const double *c, int ldc,
                                 double *mo_packed, int precision, int n_slices,

@@ -145,8 +145,10 @@ def test_host_entry_points_over_many_bounce_chunks(ctx):
     """At N_AO = N_MO = 140 the packed blocks are 390 MB, seven 64 MiB chunks of the
     host entry points' pinned bounce slots (four slots, so each is reused), and the
     label stream is 390 MB of values plus 195 MB of labels.  The host entry points
-    (packed and label stream) give the device path's bits, and the label stream is the
-    pshf stream of the same block."""
+    (packed and label stream) give the device path's bits -- the packed one with its
+    expand run behind the upload and its output copied during half 2, into a fresh and
+    into a reused output array -- and the label stream is the pshf stream of the same
+    block."""
     from cipsi_amd import formats
     n = m = 140
     print(f"[bounce] n={n}", flush=True)
@@ -157,6 +159,14 @@ def test_host_entry_points_over_many_bounce_chunks(ctx):
     ref = ctx.transform(ao, c_dev).cpu().numpy()
     ao_np = ao.cpu().numpy()
     assert np.array_equal(ctx.transform(ao_np, c), ref)
+    # the same output array twice (its pages already mapped), NaN before each call
+    buf = np.full(ref.size, np.nan)
+    for _ in range(2):
+        ctx.transform(ao_np, c, out=buf)
+        assert np.array_equal(buf, ref)
+        buf[:] = np.nan
     xx, ix = formats.labels_from_packed(ao_np, n)
     assert xx.nbytes > 4 * (64 << 20)
     assert np.array_equal(ctx.itijkl(xx, ix, c), ref)
+    ctx.itijkl(xx, ix, c, out=buf)
+    assert np.array_equal(buf, ref)
