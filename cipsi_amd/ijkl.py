@@ -87,6 +87,13 @@ def load_library(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is not built; run __graft_entry__.build() "
                            "(make -C cipsi_amd/csrc)")
+    # torch ships its own HIP runtime: when torch is installed it is loaded first, so the
+    # library binds to that runtime instead of bringing up a second one (whose context
+    # creation would then find no device)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     L.cipsi_ijkl_create.argtypes = [ctypes.POINTER(_vp), _c_int]
     L.cipsi_ijkl_destroy.argtypes = [_vp]
