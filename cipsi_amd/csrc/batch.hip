@@ -3,8 +3,8 @@
 // The reference transforms one geometry per ijkl process, the scan driver starting
 // one process per geometry (Autocip_13/autocip13.f90:525-567, ijkl.f:318).  Here the
 // whole scan is one hipGraph launch:
-//   pad_c (all)  ->  expand (all)  ->  half 1, one launch per row class
-//   ->  transpose (all)  ->  half 2, one launch per row class
+//   pad_c (all)  ->  expand (all)  ->  half 1, one launch per row class, storing H^T
+//   ->  half 2, one launch per row class
 // Every launch is grouped over the geometries: its kernels read the per-geometry
 // arguments from device tables, so the captured graph does not depend on the caller's
 // pointers and is reused by every run of the plan (a run uploads the tables, then
@@ -63,7 +63,7 @@ struct cipsi_ijkl_batch {
     void* host_tables = nullptr; // pinned staging of the tables
     hipEvent_t copied = nullptr; // last table upload done (staging reusable)
     bool copy_pending = false;
-    int64_t padc_total = 0, expand_total = 0, trans_total = 0;
+    int64_t padc_total = 0, expand_total = 0;
     // graph
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -105,7 +105,7 @@ void fill_tables(cipsi_ijkl_batch* b, const double* const* ao, const double* con
     int64_t* s1 = (int64_t*)(base + b->off_s1);
     int64_t* s2 = (int64_t*)(base + b->off_s2);
     char* work = (char*)b->work;
-    int64_t padc = 0, expd = 0, trans = 0;
+    int64_t padc = 0, expd = 0;
     for (int g = 0; g < b->ngeom; ++g) {
         LayoutGeom& L = lg[g];
         L.ao = ao[g];
@@ -122,14 +122,12 @@ void fill_tables(cipsi_ijkl_batch* b, const double* const* ao, const double* con
         L.pm = npairs(L.m);
         L.padc_start = padc;
         L.expand_start = expd;
-        L.trans_start = trans;
+        L.trans_start = 0;                       // half 1 stores H^T: no transpose
         padc += (int64_t)kCRows * L.mp;
         expd += expand_tiles(L.pa);
-        trans += transpose_tiles(L.pa, L.pm);
     }
     b->padc_total = padc;
     b->expand_total = expd;
-    b->trans_total = trans;
     const int64_t* tt = (const int64_t*)b->tile_tables;
     int slot = 0;
     for (size_t k = 0; k < b->classes.size(); ++k) {
@@ -152,8 +150,9 @@ void fill_tables(cipsi_ijkl_batch* b, const double* const* ao, const double* con
             a.src = L.vsq;
             a.src_ld = L.pa;
             a.b_begin = 0;
-            HalfArgs x = a;                      // half 1: AO pairs of the expanded block
-            x.dst = L.h;
+            HalfArgs x = a;                      // half 1: AO pairs of the expanded block,
+            x.dst = L.h;                         // stored as H^T
+            x.store_t = 1;
             x.half = 1;
             x.nb = L.pa;
             x.dst_ld = L.pm;
@@ -162,6 +161,7 @@ void fill_tables(cipsi_ijkl_batch* b, const double* const* ao, const double* con
             h1[cp.first + i] = x;
             st1[i + 1] = st1[i] + x.ntile_total;
             HalfArgs y = a;                      // half 2: MO pairs of H^T
+            y.src = L.h;
             y.dst = mo[g];
             y.half = 2;
             y.nb = L.pm;
@@ -235,7 +235,6 @@ hipError_t enqueue_all(const cipsi_ijkl_batch* b, hipStream_t st) {
     if ((e = launch_group_expand(lg, b->ngeom, b->expand_total, st)) != hipSuccess) return e;
     for (size_t k = 0; k < b->classes.size(); ++k)
         if ((e = launch_half_fp64_group(group_args(b, k, 1), st)) != hipSuccess) return e;
-    if ((e = launch_group_transpose(lg, b->ngeom, b->trans_total, st)) != hipSuccess) return e;
     for (size_t k = 0; k < b->classes.size(); ++k)
         if ((e = launch_half_fp64_group(group_args(b, k, 2), st)) != hipSuccess) return e;
     return hipSuccess;
@@ -471,13 +470,8 @@ int cipsi_ijkl_batch_autotune(cipsi_ijkl_ctx* ctx, cipsi_ijkl_batch* b, const do
             if ((rc = prepare_class(ctx, cp))) return rc;
         }
         if ((rc = upload_tables(ctx, b, d_ao, d_c, ldc, d_mo))) return rc;
-        if (half == 1) {
-            // rerun half 1 with the chosen variants (the last timed variant wrote H last,
-            // which is the same H: the variants differ only in tiling), then transpose
-            if ((e = launch_group_transpose(lg, b->ngeom, b->trans_total, ctx->stream)) !=
-                hipSuccess)
-                return fail(ctx, e, "batch autotune");
-        }
+        // after half 1 the last timed variant has written H^T, the same H^T as the chosen
+        // one (the variants differ only in tiling), which half 2's timings read
     }
     // leave the outputs of a full run with the chosen plan
     if ((e = enqueue_all(b, ctx->stream)) != hipSuccess) return fail(ctx, e, "batch autotune");

@@ -61,6 +61,8 @@ struct HalfArgs {
     int64_t dst_ld;        // half 1: P_M
     int64_t ntile_total;   // end of the (b, column tile) work items of this launch
     int64_t item_begin;    // first work item of this launch (0: the whole half)
+    int store_t;           // half 1: 1 = write H^T (dst[pair0(c, a) * nb + b]) instead of
+                           // H (dst[b * dst_ld + pair0(c, a)])
     const int64_t* tile_start;  // half 2: prefix table over l (m + 1 entries), device
 };
 

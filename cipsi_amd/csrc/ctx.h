@@ -103,10 +103,12 @@ int ctx_copy_h2d(cipsi_ijkl_ctx* ctx, void* d, const void* h, size_t bytes);
 int ctx_enter(cipsi_ijkl_ctx* ctx);
 int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int64_t* total);
 int run_pad_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_c, int ldc, double** cr);
+// store_t (FP64 half 1): write H^T, rows of P_A per MO pair, instead of H.
 int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, const double* cr,
-             double* dst, int precision, int n_slices);
+             double* dst, int precision, int n_slices, int store_t = 0);
 int run_half_variant(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src,
-                     const double* cr, double* dst, int precision, int n_slices, int var);
+                     const double* cr, double* dst, int precision, int n_slices, int var,
+                     int store_t = 0);
 int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half, int n, int m, int precision);
 
 // No-allocation transform (noalloc.hip): resolves the device address of the static

@@ -14,7 +14,9 @@
 // The D layout of a T row block is therefore exactly the operand layout of the four
 // K-steps r = 16rb + 4q + {0..3}: T never leaves the registers.  Q2 is computed as
 // O^T(c, a) = sum_r T(r, c) C(r, a) with T as the A operand, so a lane's results are
-// 16 consecutive a of one c -- 128-byte coalesced stores into the packed output.
+// 16 consecutive a of one c -- 128-byte coalesced stores into the packed output.  Half 1
+// may instead store H^T (store_t): element (a, c) of matrix b at pair0(c, a) * nb + b, one
+// 128-byte line per lane, completed in L2 by the neighbouring AO pairs of the same line.
 //
 // Only the Z_b panels (KC columns s of the symmetric matrix, all rows) go through LDS,
 // prefetched into registers one panel ahead; the C operands of both products are read
@@ -259,7 +261,10 @@ __device__ __forceinline__ void half_tile(const HalfArgs& A, int64_t item) {
             const int c = cw + lg + 4 * q;
             if (c < c1 && a <= c) {
                 const int64_t pp = pair0(c, a);
-                if (pp < limit) dst[pp] = o0[q] + o1[q];
+                if (pp < limit) {
+                    if (A.store_t) ((gdouble*)A.dst)[pp * A.nb + b] = o0[q] + o1[q];
+                    else dst[pp] = o0[q] + o1[q];
+                }
             }
         }
     }
