@@ -176,7 +176,7 @@ def write_info_molecule(path, mol, c_square: np.ndarray, h_packed: np.ndarray,
     n = mol.n_ao
     nsh = mol.nshell
     nt = mol.nt
-    grp = group or {"C2v": "CNV", "D2h": "DNH"}[mol.group]
+    grp = group or {"C2v": "CNV", "D2h": "DNH", "Cinfv": "CINFV", "Dinfh": "DINFH"}[mol.group]
     head = _i4(*([0] * 19)) + _i4(nt, nsh, n, n, 0) + _ch("rhf", 8) \
         + _i4(nsh, n_elec, 0, 1, n_elec // 2, n_elec // 2, 2) + _ch(grp, 8) + _i4(2) \
         + _r8(*([0.0] * 10))
@@ -184,7 +184,7 @@ def write_info_molecule(path, mol, c_square: np.ndarray, h_packed: np.ndarray,
     body = b"".join(_i4(kt, s + 1, mol.kloc[s] + 1, atom + 1, 1)
                     for s, (atom, kt) in enumerate(mol.shells))
     body += b"".join(_r8(1.0 + 0.5 * s, 1.0, 1.0, 1.0, 0.0, 0.0) for s in range(nsh))
-    for atom, z in ((0, 1.25), (1, -1.25)):
+    for atom, z in enumerate(mol.atom_z):
         body += _r8(0.0, 0.0, z) + _r8(0.0) + _i4(1, 1, 1) + _r8(float(n_elec) / 2)
     body += _i4(*[int(mol.newsh[s, g]) + 1 for g in range(nt) for s in range(nsh)])
     pd, dd = mol.ptr_diag()

@@ -278,17 +278,90 @@ def sym_prepare(mol_tables, s_packed, h_packed, c_square, noc, nvecf=None, ygel=
     return r
 
 
-def sym_itijkl(prep, ao_tables, xx, ix, out_cap=None):
+def sym_linear(group, mol_tables):
+    """otodon.f:36-144 for CINFV / DINFH.  mol_tables: dict with norb, ktype, kloc (per
+    shell, 1-based), zsh (z of each shell's atom) and, for DINFH, newsh (nshell x nt).
+    Returns (nsym, its (nsym x nsym), ntrans, isydeg (nsym x ntrans), so (nsym x N))."""
+    L = lib()
+    ip = ctypes.POINTER(ctypes.c_int)
+    f = L.or_sym_linear
+    f.restype = ctypes.c_int
+    t = mol_tables
+    n = t["norb"]
+    nshell = len(t["ktype"])
+    dinfh = group.upper().strip() == "DINFH"
+    if not dinfh and group.upper().strip() != "CINFV":
+        raise ValueError(group)
+    newsh2 = _i32(np.asarray(t["newsh"])[:, 1] if dinfh else np.zeros(nshell))
+    its = np.zeros(14 * 14, dtype=np.int32)
+    isydeg = np.zeros(14 * 2, dtype=np.int32)
+    so = np.zeros(14 * n, dtype=np.int32)
+    nsym, ntrans = ctypes.c_int(), ctypes.c_int()
+    p = lambda a: a.ctypes.data_as(ip)
+    f(int(dinfh), n, nshell, p(_i32(t["ktype"])), p(_i32(t["kloc"])), _d(_f64(t["zsh"])), p(newsh2),
+      ctypes.byref(nsym), p(its), ctypes.byref(ntrans), p(isydeg), p(so))
+    ns, nt = nsym.value, ntrans.value
+    return (ns, its[:ns * ns].reshape(ns, ns).T.copy(), nt,
+            isydeg[:ns * nt].reshape(nt, ns).T.copy(), so[:ns * n].reshape(ns, n).copy())
+
+
+def sym_prepare_so(norb, nsym, its, so, s_packed, h_packed, c_square, noc, nvecf=None,
+                   ygel=None, trec=1e-5, tdeg=1e-6):
+    """rotat2.f:50-423 + vector.f:17-46 for the symmetry-orbital table so (nsym x N) that
+    sym_linear returns.  Returns a SymPrep as sym_prepare."""
+    L = lib()
+    ip = ctypes.POINTER(ctypes.c_int)
+    f = L.or_sym_prepare_so
+    f.restype = ctypes.c_int
+    n = norb
+    nvecf = n if nvecf is None else nvecf
+    yg = np.zeros(n, dtype=np.uint8) if ygel is None else np.ascontiguousarray(ygel, np.uint8)
+    c_in = np.asfortranarray(c_square, dtype=np.float64)
+    c_out = np.zeros((n, n), order="F")
+    itsyv = np.zeros(n, dtype=np.int32)
+    isymp = np.zeros(nsym, dtype=np.int32)
+    nsymp = ctypes.c_int()
+    ivecsy = np.zeros(nsym * n, dtype=np.int32)
+    nbp = np.zeros(nsym, dtype=np.int32)
+    nbo = np.zeros(nsym, dtype=np.int32)
+    ind = np.zeros(nsym + 1, dtype=np.int32)
+    cpk = np.zeros(n * n)
+    p = lambda a: a.ctypes.data_as(ip)
+    rc = f(n, nsym, p(_i32(np.asarray(so).ravel())), _d(_f64(s_packed)), _d(_f64(h_packed)),
+           _d(c_in), nvecf, yg.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), noc,
+           ctypes.c_double(trec), ctypes.c_double(tdeg), _d(c_out), p(itsyv), p(isymp),
+           ctypes.byref(nsymp), p(ivecsy), p(nbp), p(nbo), p(ind), _d(cpk))
+    if rc != 0:
+        raise ValueError(rc)
+    r = SymPrep()
+    r.norb, r.nsym, r.nvec = n, nsym, nvecf
+    r.its = np.asarray(its, dtype=np.int32)
+    r.c = c_out
+    r.itsyv = itsyv[:nvecf].copy()
+    r.isymp = isymp
+    r.nsymp = nsymp.value
+    r.ivecsy = ivecsy.reshape(nsym, n)
+    r.nbp, r.nbo, r.ind = nbp, nbo, ind
+    r.c_packed = cpk[:int(ind[nsym]) - 1].copy()
+    return r
+
+
+def sym_itijkl(prep, ao_tables, xx, ix, out_cap=None, isydeg=None):
     """itijkl.f:42-298 with transd / tpqkld / tijkld in their symmetric forms.  ao_tables:
     dict with ishell, kloc, ktyp, numc (per AO, 1-based shell and kloc), newsh (nshell x nt,
-    1-based), iptr, idtr, iftr.  Returns (mo_stream, ipair (npaire x 2), nad, ktypsy)."""
+    1-based), iptr, idtr, iftr.  isydeg: the symmetry transformations (nsym x ntrsy) of
+    the linear groups, or None.  Returns (mo_stream, ipair (npaire x 2), nad, ktypsy)."""
     L = lib()
     ip = ctypes.POINTER(ctypes.c_int)
     f = L.or_sym_itijkl
     f.restype = ctypes.c_int
     t = ao_tables
-    n, nsym, nt = prep.norb, prep.nsym, prep.nt
-    nshell = np.asarray(t["newsh"]).shape[0]
+    n, nsym = prep.norb, prep.nsym
+    nshell, nt = np.asarray(t["newsh"]).shape
+    if isydeg is None:
+        isydeg = np.zeros((nsym, 0), dtype=np.int32)
+    ntrsy = int(np.asarray(isydeg).shape[1])
+    isy = _i32(np.asarray(isydeg).T.ravel()) if ntrsy else np.zeros(1, dtype=np.int32)
     its = _i32(prep.its.T.ravel())
     newsh = _i32(np.asarray(t["newsh"]).T.ravel())
     iptr = _i32(np.asarray(t["iptr"]).T.ravel())
@@ -312,7 +385,7 @@ def sym_itijkl(prep, ao_tables, xx, ix, out_cap=None):
     p = lambda a: a.ctypes.data_as(ip)
     rc = f(n, nsym, p(its), p(_i32(prep.isymp)), p(_i32(prep.nbp)), p(_i32(prep.nbo)),
            p(_i32(prep.ind)), p(ivecsy), _d(_f64(prep.c_packed)), nt, nshell, p(ishell), p(kloc),
-           p(ktyp), p(numc), p(newsh), p(iptr), p(idtr), p(iftr), _d(xx),
+           p(ktyp), p(numc), p(newsh), p(iptr), p(idtr), p(iftr), ntrsy, p(isy), _d(xx),
            ix.ctypes.data_as(_ip), xx.size, _d(out), out.size, ctypes.byref(nout), p(ipair),
            pair_cap, ctypes.byref(npaire), p(nad), p(ktypsy))
     if rc != 0:

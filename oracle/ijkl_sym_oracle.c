@@ -1,6 +1,7 @@
 /*
  * ijkl_sym_oracle.c -- CPU restatement of the reference ijkl for molecules with
- * point-group symmetry (nsym > 1, non-degenerate groups: CS, CI, CN, CNV, CNH, DN, DNH).
+ * point-group symmetry (nsym > 1: the non-degenerate groups CS, CI, CN, CNV, CNH, DN, DNH,
+ * and the linear groups CINFV, DINFH with their symmetry transformations).
  *
  * TEST INFRASTRUCTURE ONLY (the checker of the GPU path; never linked into the product).
  * Every routine follows the reference loop nests and summation order so that the
@@ -11,7 +12,11 @@
  *                    vectors, classes, degeneracy from tmono.f:37-54, projection,
  *                    normalisation and orthogonality checks, reordering by irrep),
  *                    vector.f:17-46 (per-irrep packing of C)
- *   or_sym_itijkl    itijkl.f:42-298 (class and irrep block loops, block directory),
+ *   or_sym_linear    otodon.f:36-144 (linear groups: classes by angular momentum about
+ *                    the axis, g/u combinations for DINFH, tables of BLOCKDATA1.f)
+ *   or_sym_prepare_so  rotat2 + vector for the table or_sym_linear builds
+ *   or_sym_itijkl    itijkl.f:42-298 (class and irrep block loops, the symmetry
+ *                    transformations of the linear groups, block directory),
  *                    transd.f:1-379 (label stream -> class-quartet AO block, symmetry
  *                    images), tpqkld.f:1-108 and tijkld.f:1-90 in their general
  *                    (rectangular) forms, with the unit-40 and unit-50 record streams
@@ -46,21 +51,20 @@
  *   nbp[nsym], nbo[nsym], ind[nsym+1], c_packed[...]  vector.f's /om/
  * Returns 0, or the error tag rotat2 prints before stopping (2, 1, -1).
  */
-int or_sym_prepare(int norb, int nsym, const int *its, const int *jsym, int nt, int nshell,
-                   const int *ktype, const int *kloc, const int *newsh, const int *iptr,
-                   const int *idtr, const int *iftr, const double *s_packed,
-                   const double *h_packed, const double *c_square, int nvecf,
-                   const unsigned char *ygel, int noc, double trec, double tdeg,
-                   double *c_out, int *itsyv, int *isymp, int *nsymp_out, int *ivecsy,
-                   int *nbp, int *nbo, int *ind, double *c_packed)
+static int prepare_impl(int have_so, int norb, int nsym, const int *jsym, int nt, int nshell,
+                        const int *ktype, const int *kloc, const int *newsh, const int *iptr,
+                        const int *idtr, const int *iftr, const double *s_packed,
+                        const double *h_packed, const double *c_square, int nvecf,
+                        const unsigned char *ygel, int noc, double trec, double tdeg,
+                        double *c_out, int *itsyv, int *isymp, int *nsymp_out, int *ivecsy,
+                        int *nbp, int *nbo, int *ind, double *c_packed)
 {
-    (void)its;
     const int n = norb;
     const int64_t nn = (int64_t)n * n;
     int rc = 0;
-    /* ---- otodon.f:320-353 */
-    memset(ivecsy, 0, sizeof(int) * (size_t)nsym * n);
-    {
+    /* ---- otodon.f:320-353 (have_so: ivecsy already holds the linear groups' table) */
+    if (!have_so) {
+        memset(ivecsy, 0, sizeof(int) * (size_t)nsym * n);
         int i1 = 0;
         for (int i = 1; i <= nsym; ++i) {
             const int im1n = (i - 1) * n;
@@ -317,6 +321,136 @@ int or_sym_prepare(int norb, int nsym, const int *its, const int *jsym, int nt, 
     free(c); free(csc); free(vecsym); free(a); free(srec); free(ch); free(cc);
     free(isym); free(ideg); free(iniv); free(itr);
     return rc;
+}
+
+int or_sym_prepare(int norb, int nsym, const int *its, const int *jsym, int nt, int nshell,
+                   const int *ktype, const int *kloc, const int *newsh, const int *iptr,
+                   const int *idtr, const int *iftr, const double *s_packed,
+                   const double *h_packed, const double *c_square, int nvecf,
+                   const unsigned char *ygel, int noc, double trec, double tdeg,
+                   double *c_out, int *itsyv, int *isymp, int *nsymp_out, int *ivecsy,
+                   int *nbp, int *nbo, int *ind, double *c_packed)
+{
+    (void)its;
+    return prepare_impl(0, norb, nsym, jsym, nt, nshell, ktype, kloc, newsh, iptr, idtr, iftr,
+                        s_packed, h_packed, c_square, nvecf, ygel, noc, trec, tdeg, c_out, itsyv,
+                        isymp, nsymp_out, ivecsy, nbp, nbo, ind, c_packed);
+}
+
+/* rotat2 + vector for a symmetry-orbital table so[nsym*N] that otodon built already (the
+   linear groups, or_sym_linear). */
+int or_sym_prepare_so(int norb, int nsym, const int *so, const double *s_packed,
+                      const double *h_packed, const double *c_square, int nvecf,
+                      const unsigned char *ygel, int noc, double trec, double tdeg,
+                      double *c_out, int *itsyv, int *isymp, int *nsymp_out, int *ivecsy,
+                      int *nbp, int *nbo, int *ind, double *c_packed)
+{
+    memcpy(ivecsy, so, sizeof(int) * (size_t)nsym * norb);
+    return prepare_impl(1, norb, nsym, NULL, 0, 0, NULL, NULL, NULL, NULL, NULL, NULL, s_packed,
+                        h_packed, c_square, nvecf, ygel, noc, trec, tdeg, c_out, itsyv, isymp,
+                        nsymp_out, ivecsy, nbp, nbo, ind, c_packed);
+}
+
+/* ------------------------------------------------------------------ linear groups */
+/* BLOCKDATA1.f:15-41: product tables itsdh (14 x 14) and itscv (7 x 7) and the symmetry
+   transformations isyded (14 x 2) and isydec (7 x 2), column-major as the DATA statements
+   list them. */
+static const int kItsdh[14 * 14] = {
+    1, 2, 3, 4, 5, 6, 1, 2, 9, 10, 3, 4, 5, 6,    2, 1, 4, 3, 6, 5, 2, 1, 10, 9, 4, 3, 6, 5,
+    3, 4, 1, 2, 9, 10, 3, 4, 5, 6, 1, 2, 9, 10,   4, 3, 2, 1, 10, 9, 4, 3, 6, 5, 2, 1, 10, 9,
+    5, 6, 9, 10, 1, 2, 5, 6, 3, 4, 9, 10, 1, 2,   6, 5, 10, 9, 2, 1, 6, 5, 4, 3, 10, 9, 2, 1,
+    1, 2, 3, 4, 5, 6, 1, 2, 9, 10, 3, 4, 5, 6,    2, 1, 4, 3, 6, 5, 2, 1, 10, 9, 4, 3, 6, 5,
+    9, 10, 5, 6, 3, 4, 9, 10, 1, 2, 5, 6, 3, 4,   10, 9, 6, 5, 4, 3, 10, 9, 2, 1, 6, 5, 4, 3,
+    3, 4, 1, 2, 9, 10, 3, 4, 5, 6, 1, 2, 9, 10,   4, 3, 2, 1, 10, 9, 4, 3, 6, 5, 2, 1, 10, 9,
+    5, 6, 9, 10, 1, 2, 5, 6, 3, 4, 9, 10, 1, 2,   6, 5, 10, 9, 2, 1, 6, 5, 4, 3, 10, 9, 2, 1};
+static const int kItscv[7 * 7] = {
+    1, 2, 3, 1, 5, 2, 3,   2, 1, 5, 2, 3, 1, 5,   3, 5, 1, 3, 2, 5, 1,   1, 2, 3, 1, 5, 2, 3,
+    5, 3, 2, 5, 1, 3, 2,   2, 1, 5, 2, 3, 1, 5,   3, 5, 1, 3, 2, 5, 1};
+static const int kIsyded[14 * 2] = {1, 2, 5, 6, -3, -4, -7, -8, -9, -10, 13, 14, -11, -12,
+                                    1, 2, 0, 0, 0, 0, 9, 10, -7, -8, 0, 0, 0, 0};
+static const int kIsydec[7 * 2] = {1, 3, -2, -4, -5, 7, -6,   1, 0, 0, 5, -4, 0, 0};
+/* otodon.f:37-39: component of a shell of type ktype carrying class jnk (0 = sigma,
+   1 = pi_x, 2 = pi_y, 3 = delta_x, 4 = delta_y, 5, 6 = phi), -1 if none:
+   idinf2(ktype + 4 jnk). */
+static const int kIdinf2[28] = {0, 2, 0, 0, -1, 0, 1, 1, -1, 1, 2, 2, -1, -1, 3, 3,
+                                -1, -1, 4, 4, -1, -1, -1, 5, -1, -1, -1, 6};
+
+/*
+ * otodon.f:36-144 for CINFV (dinfh = 0) and DINFH (dinfh = 1): nsym (5 or 7, 10 or 14 with
+ * f shells), the product table its (nsym x nsym, column-major), the number of symmetry
+ * transformations ntrans (one each for the p and the d shells present, starting from the
+ * 0 that ijkl.f:48 passes through rondof) with isydeg (nsym x ntrans, column-major), and
+ * the signed symmetry-orbital table so[nsym*N] (otodon's ivecsy).  Per shell: ktype,
+ * kloc (1-based first AO), zsh (the z coordinate of the shell's atom) and newsh2 (1-based
+ * image shell under the second operation of the group pshf ran in, D-inf-h only).
+ * The branch otodon.f:104-106 / 139-142 (no p shell and ntrans = 1) cannot be taken
+ * with ntrans entering as 0, so it has no counterpart here.
+ */
+int or_sym_linear(int dinfh, int norb, int nshell, const int *ktype, const int *kloc,
+                  const double *zsh, const int *newsh2, int *nsym_out, int *its,
+                  int *ntrans_out, int *isydeg, int *so)
+{
+    int yp = 0, yd = 0, yf = 0;
+    for (int i = 0; i < nshell; ++i) {
+        if (ktype[i] == 2) yp = 1;
+        if (ktype[i] == 3) yd = 1;
+        if (ktype[i] == 4) yf = 1;
+    }
+    if (yf) { yp = 1; yd = 1; }
+    if (yd) yp = 1;
+    const int ntrans = (yp ? 1 : 0) + (yd ? 1 : 0);
+    const int nsym = dinfh ? (yf ? 14 : 10) : (yf ? 7 : 5);
+    memset(so, 0, sizeof(int) * (size_t)nsym * norb);
+    if (dinfh) {
+        int yg = 0, ish = -norb;
+        for (int ink = 1; ink <= nsym; ++ink) {
+            yg = !yg;
+            ish += norb;
+            const int jnk = (ink - 1) / 2;
+            int i2 = 0;
+            for (int i = 1; i <= nshell; ++i) {
+                const int idp = kIdinf2[ktype[i - 1] + jnk * 4 - 1];
+                if (idp < 0) continue;
+                int i1 = ish + kloc[i - 1] + idp;
+                const int yki = (ktype[i - 1] - 2 * (ktype[i - 1] / 2)) == 0;
+                const int ysym = yki != yg;
+                if (!ysym && fabs(zsh[i - 1]) < 1.e-8) continue;
+                if (so[i1 - 1] != 0) continue;
+                ++i2;
+                so[i1 - 1] = i2;
+                if (fabs(zsh[i - 1]) > 1.e-8) {
+                    i1 = ish + kloc[newsh2[i - 1] - 1] + idp;
+                    if (so[i1 - 1] == 0) so[i1 - 1] = ysym ? i2 : -i2;
+                }
+            }
+        }
+    } else {
+        int ish = -norb;
+        for (int ink = 1; ink <= nsym; ++ink) {
+            const int jnk = ink - 1;
+            int i2 = 0;
+            ish += norb;
+            for (int i = 1; i <= nshell; ++i) {
+                const int idp = kIdinf2[ktype[i - 1] + jnk * 4 - 1];
+                if (idp < 0) continue;
+                const int i1 = ish + kloc[i - 1] + idp;
+                if (so[i1 - 1] != 0) continue;
+                ++i2;
+                so[i1 - 1] = i2;
+            }
+        }
+    }
+    const int *tab = dinfh ? kItsdh : kItscv;
+    const int *deg = dinfh ? kIsyded : kIsydec;
+    const int ld = dinfh ? 14 : 7;
+    for (int j = 0; j < nsym; ++j) {
+        for (int i = 0; i < nsym; ++i) its[i + j * nsym] = tab[i + j * ld];
+    }
+    for (int t = 0; t < ntrans; ++t)
+        for (int j = 0; j < nsym; ++j) isydeg[j + t * nsym] = deg[j + t * ld];
+    *nsym_out = nsym;
+    *ntrans_out = ntrans;
+    return 0;
 }
 
 /* ------------------------------------------------------------------ itijkl and its routines */
@@ -728,8 +862,21 @@ static void sym_tijkld(SymState *S, int mi, int mj, int mk, int ml, int ic, int 
     }
 }
 
+/* itijkl.f:28-36: ltabs(nklst, ktypsy), the type of a block that a symmetry
+   transformation maps onto an earlier block (column-major as the DATA statement). */
+static const int kLtabs[100] = {
+    1, 1, 1, 1, 1, 1, 1, 1, 1, 10,   1, 2, 3, 4, 5, 6, 7, 8, 9, 1,
+    1, 3, 2, 5, 4, 7, 6, 9, 8, 1,    1, 4, 5, 2, 3, 8, 9, 6, 7, 1,
+    1, 5, 4, 3, 2, 9, 8, 7, 6, 1,    1, 6, 7, 8, 9, 2, 3, 4, 5, 1,
+    1, 7, 6, 9, 8, 3, 2, 5, 4, 1,    1, 8, 9, 6, 7, 4, 5, 2, 3, 1,
+    1, 9, 8, 7, 6, 5, 4, 3, 2, 1,    10, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+
 /*
- * itijkl.f:42-298 for nsym > 1 without degenerate-irrep equivalences (ntrsy = 0).
+ * itijkl.f:42-298 for nsym > 1.  With ntrsy > 0 (the linear groups), the symmetry
+ * transformations isydeg (nsym x ntrsy, column-major) map irrep quartets onto earlier
+ * ones (itijkl.f:121-200): such a block is not computed, its directory entry points at
+ * the earlier block with a sign and a permutation type, and a block the transformation
+ * maps onto itself with the pairs exchanged is stored as a triangle.
  * Inputs: the /om/ and /hondo/ state after or_sym_prepare and rondof (ishell, kloc,
  * ktyp, numc per AO, 1-based; newsh per shell and operation), the label stream.
  * Outputs: out[] the f50 MO stream in tijkld's order (up to out_cap values; *n_out gets
@@ -741,9 +888,9 @@ int or_sym_itijkl(int norb, int nsym, const int *its, const int *isymp, const in
                   const int *nbo, const int *ind, const int *ivecsy, const double *c_packed,
                   int nt, int nshell, const int *ishell, const int *kloc, const int *ktyp,
                   const int *numc, const int *newsh, const int *iptr, const int *idtr,
-                  const int *iftr, const double *xx, const int32_t *ix, int64_t nlab,
-                  double *out, int64_t out_cap, int64_t *n_out, int *ipair, int pair_cap,
-                  int *npaire, int *nad, int *ktypsy)
+                  const int *iftr, int ntrsy, const int *isydeg, const double *xx,
+                  const int32_t *ix, int64_t nlab, double *out, int64_t out_cap, int64_t *n_out,
+                  int *ipair, int pair_cap, int *npaire, int *nad, int *ktypsy)
 {
     SymState S;
     memset(&S, 0, sizeof(S));
@@ -772,6 +919,7 @@ int or_sym_itijkl(int norb, int nsym, const int *its, const int *isymp, const in
     int rc = 0;
     if (!blk || !tab || !tac || !rec) rc = -1;
 #define ITS(a, b) its[((a) - 1) + ((b) - 1) * nsym]
+#define ISYDEG(a, t) isydeg[((a) - 1) + ((t) - 1) * nsym]
     for (int msi = 1; msi <= nsymp && !rc; ++msi)
         for (int msj = 1; msj <= nsymp && !rc; ++msj) {
             const int ypq = msi == msj;
@@ -803,7 +951,7 @@ int or_sym_itijkl(int norb, int nsym, const int *its, const int *isymp, const in
                                     const int ykl = kk == ll;
                                     int64_t nijkl = mono1 * mono2;
                                     if (ypqrs) nijkl = (mono1 + nijkl) / 2;
-                                    const int yijkl = ii == kk && jj == ll;
+                                    int yijkl = ii == kk && jj == ll;
                                     const int kls = ITS(kk, ll);
                                     if (ITS(kls, ijs) != 1) continue;
                                     const int nkl = kk * (kk - 1) / 2 + ll;
@@ -812,6 +960,49 @@ int or_sym_itijkl(int norb, int nsym, const int *its, const int *isymp, const in
                                     const int ijkls = nij * (nij - 1) / 2 + nkl;
                                     int64_t wijkl = mkl * mij;
                                     if (wijkl <= 0) continue;
+                                    /* itijkl.f:121-200: symmetry transformations */
+                                    int skip = 0;
+                                    for (int t = 1; t <= ntrsy && !skip; ++t) {
+                                        int isig = 1;
+                                        int iit = ISYDEG(ii, t), jjt, kkt, llt;
+                                        if (iit == 0) continue;
+                                        if (iit < 0) { iit = -iit; isig = -1; }
+                                        jjt = ISYDEG(jj, t);
+                                        if (jjt == 0) continue;
+                                        if (jjt < 0) { jjt = -jjt; isig = -isig; }
+                                        kkt = ISYDEG(kk, t);
+                                        if (kkt == 0) continue;
+                                        if (kkt < 0) { kkt = -kkt; isig = -isig; }
+                                        llt = ISYDEG(ll, t);
+                                        if (llt == 0) continue;
+                                        if (llt < 0) { llt = -llt; isig = -isig; }
+                                        int nklst = 2;                       /* 180 */
+                                        if (iit == kk && jjt == ll && kkt == ii && llt == jj) yijkl = 1;
+                                        if (iit < jjt) { int x = iit; iit = jjt; jjt = x; nklst += 1; }
+                                        if (kkt < llt) { int x = kkt; kkt = llt; llt = x; nklst += 2; }
+                                        if (iit < kkt || (iit == kkt && llt > jjt)) {    /* 182-184 */
+                                            int x = iit; iit = kkt; kkt = x;
+                                            x = jjt; jjt = llt; llt = x;
+                                            nklst += 4;
+                                        }
+                                        const int ijt = iit * (iit - 1) / 2 + jjt;   /* 185 */
+                                        const int klt = kkt * (kkt - 1) / 2 + llt;
+                                        if (ijt > nij || (ijt == nij && klt >= nkl)) {  /* 186 */
+                                            if (ITS(ITS(iit, jjt), ITS(kkt, llt)) != 1) skip = 1;
+                                            continue;
+                                        }
+                                        const int ijklt = ijt * (ijt - 1) / 2 + klt;     /* 187 */
+                                        if (ijkls > nsyt) { rc = -1; break; }
+                                        if (ktypsy[ijklt - 1] == 10 || ktypsy[ijklt - 1] == 1) {
+                                            if (nklst == 5) nklst = 10;
+                                            if (nklst != 10) nklst = 1;
+                                        }
+                                        ktypsy[ijkls - 1] = kLtabs[(nklst - 1) + 10 * (ktypsy[ijklt - 1] - 1)];
+                                        nad[ijkls - 1] = isig * nad[ijklt - 1];
+                                        skip = 1;
+                                    }
+                                    if (rc) break;
+                                    if (skip) continue;
                                     if (yr) {
                                         memset(blk, 0, sizeof(double) * (size_t)nijkl);
                                         const int64_t last = sym_transd(&S, ii, jj, kk, ll, blk);
@@ -855,6 +1046,7 @@ int or_sym_itijkl(int norb, int nsym, const int *its, const int *isymp, const in
                 }
         }
 #undef ITS
+#undef ISYDEG
     *npaire = npair / 2;
     *n_out = S.nout;
     if (S.oom) rc = -1;

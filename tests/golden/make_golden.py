@@ -10,7 +10,8 @@ The AO integrals are regenerated from the seed by cipsi_amd.synth (exact by
 construction, so identical on every machine); a SHA-256 of them is stored and
 checked when the fixture is used.
 
-Symmetric cases (point groups C2v and D2h, synth_sym) store the reference's whole f50
+Symmetric cases (point groups C2v and D2h, and the linear groups C-inf-v and D-inf-h,
+synth_sym) store the reference's whole f50
 stream (all symmetry blocks, in its record order), f25 and ijom bytes; their inputs C
 and h are stored too, the AO integrals and the skeleton label stream are regenerated
 from the seed (exact by construction, SHA-256 checked).
@@ -160,12 +161,63 @@ def make_sym_case(name):
           f"({os.path.getsize(path) / 1024:.0f} KiB)")
 
 
+# Linear molecules (synth_sym.make_linear_case): name -> (group, shells of atom 1, shells
+# of atom 2 (None: homonuclear), rank, seed, occupied MOs).  The info file names CINFV /
+# DINFH and carries the operations of C1 / Cs, as pshf writes them.
+LIN_CASES = {
+    "lin_cinfv_n14": ("Cinfv", [1, 2, 3, 1], [1, 2], 2, 5, 3),
+    "lin_dinfh_n18": ("Dinfh", [1, 2, 3], None, 2, 5, 3),
+    "lin_cinfv_sp_n9": ("Cinfv", [1, 1, 2], [1, 2], 2, 7, 2),
+    "lin_dinfh_sp_n10": ("Dinfh", [1, 1, 2], None, 2, 9, 3),
+    "lin_dinfh_s_n4": ("Dinfh", [1, 1], None, 2, 3, 1),
+    "lin_cinfv_n28": ("Cinfv", [1, 1, 2, 2, 3, 3], [1, 1, 2, 3], 2, 17, 6),
+    "lin_dinfh_n36": ("Dinfh", [1, 1, 2, 2, 3, 3], None, 2, 19, 7),
+}
+
+
+def make_lin_case(name):
+    from cipsi_amd import synth_sym
+    grp, sa, sb, rank, seed, noc = LIN_CASES[name]
+    case = synth_sym.make_linear_case(grp, sa, sb, rank=rank, seed=seed)
+    mol = case["mol"]
+    with tempfile.TemporaryDirectory(prefix="ijkl_ref_lin_") as wd:
+        formats.write_info_molecule(os.path.join(wd, "info"), mol, case["c_square"],
+                                    case["h_packed"], case["s_packed"], enuc=1.5,
+                                    n_elec=2 * noc)
+        nrec = formats.write_pqrs(os.path.join(wd, "pqrs"), case["xx"], case["ix"])
+        stdin = formats.ijfil_namelist() + formats.ijklin_namelist(nrec)
+        t0 = time.time()
+        res = subprocess.run([REF_BIN], input=stdin.encode(), cwd=wd, capture_output=True)
+        dt = time.time() - t0
+        log = res.stdout.decode(errors="replace") + res.stderr.decode(errors="replace")
+        if res.returncode != 0 or "fin ijkl" not in log:
+            raise RuntimeError("reference ijkl failed:\n" + log[-3000:])
+        recs = formats.read_records(os.path.join(wd, "f50"))
+        with open(os.path.join(wd, "f25"), "rb") as f:
+            f25_bytes = f.read()
+        ijom = formats.read_records(os.path.join(wd, "ijom"))
+    stream = np.concatenate([np.frombuffer(r, "<f8") for r in recs[:-1]])
+    out = dict(group=grp, shells_a=np.array(sa), shells_b=np.array(sb if sb else []),
+               rank=rank, seed=seed, noc=noc, nrec=nrec, ref_seconds=dt,
+               c_square=case["c_square"], h_packed=case["h_packed"],
+               ao_sha=sha(case["ao_packed"]), xx_sha=sha(case["xx"]), labels_sha=sha(case["ix"]),
+               f50_stream=stream, n_f50_records=len(recs) - 1, hmo=np.frombuffer(recs[-1], "<f8"),
+               f25_bytes=np.frombuffer(f25_bytes, dtype=np.uint8),
+               ijom_c=np.frombuffer(ijom[0], "<f8"), ijom_s=np.frombuffer(ijom[2], "<f8"))
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: N={mol.n_ao} NREC={nrec} ref {dt:.2f}s f50 records={len(recs) - 1} -> {path} "
+          f"({os.path.getsize(path) / 1024:.0f} KiB)")
+
+
 def main(argv):
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"], check=True)
-    names = argv or list(CASES) + list(SYM_CASES)
+    names = argv or list(CASES) + list(SYM_CASES) + list(LIN_CASES)
     for n in names:
         if n in SYM_CASES:
             make_sym_case(n)
+        elif n in LIN_CASES:
+            make_lin_case(n)
         else:
             make_case(n)
 

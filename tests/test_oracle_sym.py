@@ -25,14 +25,22 @@ from tests.conftest import load_golden
 
 SYM = ["sym_c2v_n19", "sym_c2v_deg_n19", "sym_d2h_n20", "sym_d2h_n24", "sym_c2v_n30",
        "sym_d2h_n38"]
+# linear molecules: C-inf-v and D-inf-h with p and d shells (two symmetry transformations),
+# with s and p shells only (one), with s shells only (none)
+LIN = ["lin_cinfv_n14", "lin_dinfh_n18", "lin_cinfv_sp_n9", "lin_dinfh_sp_n10",
+       "lin_dinfh_s_n4", "lin_cinfv_n28", "lin_dinfh_n36"]
 
 
 def regenerate(g):
     import hashlib
     sb = list(g["shells_b"]) if g["shells_b"].size else None
-    case = synth_sym.make_sym_case(str(g["group"]), list(g["shells_a"]), sb,
-                                   rank_per_irrep=int(g["rank"]), seed=int(g["seed"]),
-                                   degenerate_pair=bool(g["degenerate"]))
+    if str(g["group"]) in synth_sym.LINEAR:
+        case = synth_sym.make_linear_case(str(g["group"]), list(g["shells_a"]), sb,
+                                          rank=int(g["rank"]), seed=int(g["seed"]))
+    else:
+        case = synth_sym.make_sym_case(str(g["group"]), list(g["shells_a"]), sb,
+                                       rank_per_irrep=int(g["rank"]), seed=int(g["seed"]),
+                                       degenerate_pair=bool(g["degenerate"]))
     sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
     assert sha(case["ao_packed"]) == str(g["ao_sha"])
     assert sha(case["xx"]) == str(g["xx_sha"]) and sha(case["ix"]) == str(g["labels_sha"])
@@ -44,13 +52,28 @@ def ao_tables(t):
                 newsh=t["newsh"], iptr=t["iptr"], idtr=t["idtr"], iftr=t["iftr"])
 
 
-def run_oracle(case, g, tables=None):
+def linear_tables(case):
+    """otodon's linear-group tables from the oracle: (nsym, its, isydeg, so)."""
+    t = synth_sym.tables(case["mol"])
+    nsym, its, _, isydeg, so = oracle.sym_linear(synth_sym.GROUP_NAME[case["mol"].group], t)
+    return nsym, its, isydeg, so
+
+
+def run_oracle(case, g, tables=None, with_isydeg=False):
     t = tables or synth_sym.tables(case["mol"])
-    prep = oracle.sym_prepare(t, np.eye(case["n_ao"])[np.tril_indices(case["n_ao"])],
-                              g["h_packed"], g["c_square"], int(g["noc"]))
+    n = case["n_ao"]
+    s_unit = np.eye(n)[np.tril_indices(n)]
+    isydeg = None
+    if case["mol"].linear:
+        nsym, its, isydeg, so = linear_tables(case)
+        prep = oracle.sym_prepare_so(n, nsym, its, so, s_unit, g["h_packed"], g["c_square"],
+                                     int(g["noc"]))
+    else:
+        prep = oracle.sym_prepare(t, s_unit, g["h_packed"], g["c_square"], int(g["noc"]))
     xx = np.append(case["xx"], 0.0)
     ix = np.append(case["ix"], 0).astype(np.int32)
-    return prep, oracle.sym_itijkl(prep, ao_tables(t), xx, ix)
+    res = oracle.sym_itijkl(prep, ao_tables(t), xx, ix, isydeg=isydeg)
+    return (prep, res, isydeg) if with_isydeg else (prep, res)
 
 
 def f25_records(g):
@@ -65,8 +88,18 @@ def f25_records(g):
         os.unlink(path)
 
 
-def closed_form_stream(case, prep, ipair):
-    """The f50 stream from the dense MO integrals, block by block in directory order."""
+def split_pair(x):
+    """nij = ii (ii - 1) / 2 + jj -> (ii, jj)."""
+    ii = int((1 + np.sqrt(8 * x - 7)) // 2)
+    while ii * (ii - 1) // 2 >= x:
+        ii -= 1
+    while (ii + 1) * ii // 2 < x:
+        ii += 1
+    return ii, x - ii * (ii - 1) // 2
+
+
+def mo_integrals(case, prep):
+    """(pq|rs) over the MOs rotat2 leaves, from the factors: sum_P Bt^P_pq Bt^P_rs."""
     n = case["n_ao"]
     ia, ib = np.tril_indices(n)
     bt = []
@@ -76,20 +109,23 @@ def closed_form_stream(case, prep, ipair):
         b[ib, ia] = case["B"][P]
         bt.append(prep.c.T @ b @ prep.c)
     bt = np.array(bt)
-    g4 = np.einsum("Ppq,Prs->pqrs", bt, bt)
+    return np.einsum("Ppq,Prs->pqrs", bt, bt)
+
+
+def closed_form_stream(case, prep, ipair, ktypsy=None):
+    """The f50 stream from the dense MO integrals, block by block in directory order.  A
+    block is stored as a triangle when its pairs coincide or, with the linear groups'
+    transformations, when ktypsy marks it 1."""
+    g4 = mo_integrals(case, prep)
     mos = [np.flatnonzero(prep.itsyv == s + 1) for s in range(prep.nsym)]
     out = []
     for nij, nkl in ipair:
-        ii = int((1 + np.sqrt(8 * nij - 7)) // 2)
-        while ii * (ii - 1) // 2 >= nij:
-            ii -= 1
-        jj = nij - ii * (ii - 1) // 2
-        kk = int((1 + np.sqrt(8 * nkl - 7)) // 2)
-        while kk * (kk - 1) // 2 >= nkl:
-            kk -= 1
-        ll = nkl - kk * (kk - 1) // 2
+        ii, jj = split_pair(nij)
+        kk, ll = split_pair(nkl)
         A, B, C, D = mos[ii - 1], mos[jj - 1], mos[kk - 1], mos[ll - 1]
         diag = ii == kk and jj == ll
+        if ktypsy is not None:
+            diag = ktypsy[nij * (nij - 1) // 2 + nkl - 1] == 1
         ilk = 0
         for l in range(A.size):
             for k in range(l + 1 if ii == jj else B.size):
@@ -176,3 +212,162 @@ def test_planted_image_sign_fails():
     cf = closed_form_stream(case, prep, ipair)
     assert cf.size == stream.size
     assert np.max(np.abs(cf - stream)) > 1e-6
+
+
+# ----------------------------------------------------------------------- linear groups
+@pytest.mark.parametrize("name", LIN)
+def test_lin_oracle_bitexact_against_reference(name):
+    """C-inf-v / D-inf-h: otodon's tables from the basis (its and the isydeg records of
+    f25), rotat2's coefficients, the f50 stream without the blocks the symmetry
+    transformations make redundant, and the directory with its signed pointers (nad) and
+    block types (ktypsy), all equal to the reference's bytes."""
+    g = load_golden(name)
+    case = regenerate(g)
+    prep, (stream, ipair, nad, ktypsy), isydeg = run_oracle(case, g, with_isydeg=True)
+    assert np.array_equal(prep.c.ravel(order="F"), g["ijom_c"])
+    recs = f25_records(g)
+    nsym, nvec, noc, ntrans = struct.unpack_from("<4i", recs[0], 0)
+    assert nsym == prep.nsym == (10 if str(g["group"]) == "Dinfh" else 5)
+    assert ntrans == isydeg.shape[1]
+    assert np.array_equal(np.frombuffer(recs[0][16:16 + 4 * nvec], "<i4"), prep.itsyv)
+    assert np.array_equal(np.frombuffer(recs[1], "<i4").reshape(nsym, nsym).T, prep.its)
+    for t in range(ntrans):                                 # ijkl.f:254-261
+        r = np.frombuffer(recs[2 + t], "<i4")
+        assert r[0] == 1 and np.array_equal(r[1:], isydeg[:, t])
+    o = 2 + ntrans
+    ref = g["f50_stream"]
+    assert stream.size <= ref.size < stream.size + 8192
+    assert np.array_equal(stream, ref[:stream.size])
+    assert struct.unpack_from("<i", recs[o], 0)[0] == ipair.shape[0]
+    assert np.array_equal(np.frombuffer(recs[o][4:], "<i4").reshape(-1, 2), ipair)
+    nsyt = struct.unpack_from("<i", recs[o + 1], 0)[0]
+    assert np.array_equal(np.frombuffer(recs[o + 1][4:], "<i2"), nad[:nsyt])
+    assert np.array_equal(np.frombuffer(recs[o + 2], "<i2"), ktypsy[:nsyt])
+    assert np.array_equal(oracle.tmono(g["h_packed"], prep.c), g["hmo"])
+
+
+@pytest.mark.parametrize("name", LIN)
+def test_lin_stream_matches_closed_form(name):
+    g = load_golden(name)
+    case = regenerate(g)
+    prep, (stream, ipair, _, ktypsy) = run_oracle(case, g)
+    cf = closed_form_stream(case, prep, ipair, ktypsy)
+    assert cf.size == stream.size
+    assert np.max(np.abs(cf - stream)) <= 1e-12
+
+
+def _transformation_closure(isydeg):
+    """The signed irrep maps generated by the columns of isydeg (0 = not mapped), with the
+    identity, closed under composition."""
+    nsym = isydeg.shape[0]
+    maps = {tuple(range(1, nsym + 1))}
+    gens = [tuple(int(x) for x in isydeg[:, t]) for t in range(isydeg.shape[1])]
+    frontier = list(maps)
+    while frontier:
+        nxt = []
+        for m in frontier:
+            for gm in gens:
+                c = []
+                for x in m:
+                    y = gm[abs(x) - 1] if x != 0 else 0
+                    c.append(0 if y == 0 else (1 if x > 0 else -1) * y)
+                c = tuple(c)
+                if c not in maps:
+                    maps.add(c)
+                    nxt.append(c)
+        frontier = nxt
+    return sorted(maps)
+
+
+@pytest.mark.parametrize("name", ["lin_cinfv_n14", "lin_dinfh_n18", "lin_cinfv_n28"])
+def test_lin_transformations_are_symmetries(name):
+    """Mathematics behind isydeg and nad: each transformation maps the k-th MO of an irrep
+    onto +- the k-th MO of its image irrep, so (ij|kl) = s (Ti Tj|Tk Tl) with s the
+    product of the signs (checked on every irrep quartet the transformation maps).  Every
+    block the directory leaves out equals sign(nad) times the integrals of the computed
+    block |nad| under one transformation of the group they generate."""
+    g = load_golden(name)
+    case = regenerate(g)
+    prep, (_, ipair, nad, _), isydeg = run_oracle(case, g, with_isydeg=True)
+    g4 = mo_integrals(case, prep)
+    nsym = prep.nsym
+    mos = [np.flatnonzero(prep.itsyv == s + 1) for s in range(nsym)]
+    checked = 0
+    for t in range(isydeg.shape[1]):
+        m = isydeg[:, t]
+        for q in np.ndindex(nsym, nsym, nsym, nsym):
+            img = [m[x] for x in q]
+            if 0 in img or any(mos[x].size == 0 for x in q):
+                continue
+            if any(mos[x].size != mos[abs(y) - 1].size for x, y in zip(q, img)):
+                continue
+            s = int(np.prod(np.sign(img)))
+            src = g4[np.ix_(*[mos[x] for x in q])]
+            dst = g4[np.ix_(*[mos[abs(y) - 1] for y in img])]
+            assert np.max(np.abs(src - s * dst)) <= 1e-12, (t, q)
+            checked += 1
+    assert checked > 0
+    bad, skipped = _skipped_block_failures(g4, mos, ipair, nad, isydeg)
+    assert skipped > 0 and bad == []
+
+
+def _skipped_block_failures(g4, mos, ipair, nad, isydeg):
+    """The directory entries of skipped blocks whose sign(nad) and target block no
+    transformation explains, and the number of skipped blocks."""
+    nsym = len(mos)
+    computed = {(int(a), int(b)): k + 1 for k, (a, b) in enumerate(ipair)}
+    maps = _transformation_closure(isydeg)
+    skipped = 0
+    bad = []
+    for nij in range(1, nsym * (nsym + 1) // 2 + 1):
+        for nkl in range(1, nij + 1):
+            ijkls = nij * (nij - 1) // 2 + nkl
+            if ijkls > nad.size or nad[ijkls - 1] == 0 or (nij, nkl) in computed:
+                continue
+            b = abs(int(nad[ijkls - 1]))
+            tgt = tuple(split_pair(int(x)) for x in ipair[b - 1])
+            q = split_pair(nij) + split_pair(nkl)
+            src = g4[np.ix_(*[mos[x - 1] for x in q])]
+            ok = False
+            for mp in maps:
+                img = [mp[x - 1] for x in q]
+                if 0 in img:
+                    continue
+                a, bb, c, d = (abs(y) for y in img)
+                pairs = {tuple(sorted((a, bb), reverse=True)), tuple(sorted((c, d), reverse=True))}
+                if pairs != set(tgt) and not (len(set(tgt)) == 1 and pairs == set(tgt)):
+                    continue
+                s = int(np.prod(np.sign(img)))
+                dst = g4[np.ix_(*[mos[abs(y) - 1] for y in img])]
+                if s * np.sign(nad[ijkls - 1]) > 0 and np.max(np.abs(src - s * dst)) <= 1e-12:
+                    ok = True
+                    break
+            if not ok:
+                bad.append((nij, nkl, int(nad[ijkls - 1])))
+            skipped += 1
+    return bad, skipped
+
+
+def test_lin_planted_transformation_sign_fails():
+    """Flip the sign of delta_x -> -delta_x in the first transformation (the rotation by
+    90 degrees) handed to the oracle's itijkl: the sign it writes into nad for a block
+    such as (delta_x sigma|pi_y pi_y), which it skips as the image of (delta_x sigma|pi_x
+    pi_x), changes, and the mathematical check of those entries fails."""
+    g = load_golden("lin_cinfv_n14")
+    case = regenerate(g)
+    t = synth_sym.tables(case["mol"])
+    n = case["n_ao"]
+    nsym, its, isydeg, so = linear_tables(case)
+    prep = oracle.sym_prepare_so(n, nsym, its, so, np.eye(n)[np.tril_indices(n)], g["h_packed"],
+                                 g["c_square"], int(g["noc"]))
+    xx = np.append(case["xx"], 0.0)
+    ix = np.append(case["ix"], 0).astype(np.int32)
+    g4 = mo_integrals(case, prep)
+    mos = [np.flatnonzero(prep.itsyv == s + 1) for s in range(nsym)]
+    _, ipair, nad, _ = oracle.sym_itijkl(prep, ao_tables(t), xx, ix, isydeg=isydeg)
+    assert _skipped_block_failures(g4, mos, ipair, nad, isydeg)[0] == []
+    planted = isydeg.copy()
+    assert planted[3, 0] == -4
+    planted[3, 0] = 4
+    _, ipair2, nad2, _ = oracle.sym_itijkl(prep, ao_tables(t), xx, ix, isydeg=planted)
+    assert _skipped_block_failures(g4, mos, ipair2, nad2, isydeg)[0] != []
