@@ -23,8 +23,13 @@
 //   half 2   per MO pair (l,k): Y(p, q) = H[(p,q)][(l,k)]; O = C_D^T Y C_C, the triangle
 //            i <= j when C == D, rows of a diagonal block stopping at nij = ilk
 //            (tijkld.f:28-79), written at the block's offset of the output stream.
+// The linear groups add symmetry transformations (rotations about the axis that map
+// irreps onto irreps, itijkl.f:121-200): an irrep quartet that one of them maps onto an
+// earlier quartet is not transformed at all, its directory entry points at the earlier
+// block, and a quartet mapped onto itself with its pairs exchanged is stored as a
+// triangle, as a diagonal block is.  This is host bookkeeping in the block list.
 // Classes must be disjoint (every AO in the irreps of one class only), which holds for
-// diatomics in C2v and D2h; the entry point refuses anything else.
+// diatomics in C2v, D2h, C-inf-v and D-inf-h; the entry point refuses anything else.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,8 +46,10 @@ namespace cipsi {
 constexpr int kSymMaxAO = 255;
 constexpr int kSymMaxShell = 255;
 constexpr int kSymMaxOps = 8;
-constexpr int kSymMaxIrreps = 8;
-constexpr int kSymMaxClassKeys = 666;     // pairs of pairs of 8 classes
+constexpr int kSymMaxIrreps = 14;         // D-inf-h with f shells
+constexpr int kSymMaxClassPairs = kSymMaxIrreps * (kSymMaxIrreps + 1) / 2;
+constexpr int kSymMaxClassKeys = kSymMaxClassPairs * (kSymMaxClassPairs + 1) / 2;
+constexpr int kSymMaxTrans = 64;          // symmetry transformations (nsymz columns)
 
 // Basis and group tables, 0-based (uploaded once per call).
 struct SymTables {
@@ -229,7 +236,7 @@ struct SymHalfDesc {
     int pn, offd, offc;
     int tri_out;
     int half;              // 1: row b at b * dst_ld, limit dst_ld; 2: as below
-    int diag;              // half 2: diagonal block, row b at pair0(b, 0), limit b + 1;
+    int diag;              // half 2: triangular block, row b holds min(b + 1, nij) values;
                            // otherwise row b at b * nij, limit nij
     int64_t nij;
 };
@@ -338,8 +345,9 @@ __device__ __forceinline__ void sym_tile(const SymHalfDesc& D, int64_t local) {
         rowoff = b * D.dst_ld;
         limit = D.dst_ld;
     } else if (D.diag) {
-        rowoff = pair0(b, 0);
-        limit = b + 1;
+        // row b holds min(b + 1, nij) values (tijkld.f:75-77 stops at nij = ilk)
+        rowoff = b < D.nij ? pair0(b, 0) : pair0(D.nij, 0) + (b - D.nij) * D.nij;
+        limit = b < D.nij ? b + 1 : D.nij;
     } else {
         rowoff = b * D.nij;
         limit = D.nij;
@@ -440,6 +448,16 @@ int fail(cipsi_ijkl_ctx* ctx, int rc, const std::string& why) {
 inline int64_t npr(int64_t n) { return n * (n + 1) / 2; }
 inline int64_t even(int64_t x) { return (x + 31) & ~int64_t(31); }
 
+// itijkl.f:28-36 (its table ltabs): the type of a block reached through a transformation
+// whose index permutation is perm (2 + bit 0: i <-> j, bit 1: k <-> l, bit 2: the pairs
+// exchanged) from a block of type kt.  Types 2..9 compose as the permutations they stand
+// for; 1 and 10 (blocks stored as triangles) absorb them, 1 with 10 giving 10.
+inline int block_type(int perm, int kt) {
+    if (perm == 1 || perm == 10 || kt == 1 || kt == 10)
+        return (perm == 1 && kt == 10) || (perm == 10 && kt == 1) ? 10 : 1;
+    return (((perm - 2) ^ (kt - 2)) & 7) + 2;
+}
+
 }  // namespace
 
 extern "C" int cipsi_ijkl_sym_transform_labels(
@@ -457,7 +475,13 @@ extern "C" int cipsi_ijkl_sym_transform_labels(
         return CIPSI_ERR_ARG;
     if (n > kSymMaxAO) return fail(ctx, CIPSI_ERR_LABEL, "sym: N_AO > 255 (8-bit label fields)");
     if (nsym > kSymMaxIrreps || nt > kSymMaxOps || nshell > kSymMaxShell)
-        return fail(ctx, CIPSI_ERR_UNSUPPORTED, "sym: more than 8 irreps or operations");
+        return fail(ctx, CIPSI_ERR_UNSUPPORTED, "sym: more than 14 irreps or 8 operations");
+    const int ntrsy = sym->ntrsy;
+    if (ntrsy < 0 || ntrsy > kSymMaxTrans || (ntrsy > 0 && !sym->isydeg))
+        return fail(ctx, CIPSI_ERR_ARG, "sym: bad ntrsy / isydeg");
+    for (int k = 0; k < nsym * ntrsy; ++k)
+        if (sym->isydeg[k] < -nsym || sym->isydeg[k] > nsym)
+            return fail(ctx, CIPSI_ERR_ARG, "sym: an isydeg entry is not a signed irrep");
     if (n_entries >= (int64_t)1 << 60) return CIPSI_ERR_ARG;
     // ---- classes, AO subsets (vector.f numbering), the class-ordered AO basis
     const int* isymp = sym->isymp;
@@ -621,12 +645,51 @@ extern "C" int cipsi_ijkl_sym_transform_labels(
                                         isymp[kk - 1] != msk || isymp[ll - 1] != msl)
                                         continue;
                                     if (mono1 <= 0 || mono2 <= 0) continue;
-                                    const bool ykl = kk == ll, yijkl = ii == kk && jj == ll;
+                                    const bool ykl = kk == ll;
+                                    bool yijkl = ii == kk && jj == ll;
                                     if (its(its(kk, ll), its(ii, jj)) != 1) continue;
                                     const int nkl = kk * (kk - 1) / 2 + ll;
                                     int64_t mkl = (int64_t)nbo[kk - 1] * nbo[ll - 1];
                                     if (ykl) mkl = (mkl + nbo[kk - 1]) / 2;
                                     if (mkl * mij <= 0) continue;
+                                    const int ijkls = nij * (nij - 1) / 2 + nkl;
+                                    // itijkl.f:121-200: the transformations' images of the
+                                    // quartet, canonically ordered; one that comes earlier
+                                    // makes this block redundant (entry -> that block's),
+                                    // one that comes later and is forbidden, forbidden
+                                    bool skip = false;
+                                    for (int t = 0; t < ntrsy && !skip; ++t) {
+                                        const int* map = sym->isydeg + (size_t)t * nsym;
+                                        int q[4] = {map[ii - 1], map[jj - 1], map[kk - 1], map[ll - 1]};
+                                        if (!q[0] || !q[1] || !q[2] || !q[3]) continue;
+                                        int sgn = 1;
+                                        for (int& x : q)
+                                            if (x < 0) { x = -x; sgn = -sgn; }
+                                        if (q[0] == kk && q[1] == ll && q[2] == ii && q[3] == jj) yijkl = true;
+                                        int perm = 2;
+                                        if (q[0] < q[1]) { std::swap(q[0], q[1]); perm += 1; }
+                                        if (q[2] < q[3]) { std::swap(q[2], q[3]); perm += 2; }
+                                        if (q[0] < q[2] || (q[0] == q[2] && q[3] > q[1])) {
+                                            std::swap(q[0], q[2]);
+                                            std::swap(q[1], q[3]);
+                                            perm += 4;
+                                        }
+                                        const int ijt = q[0] * (q[0] - 1) / 2 + q[1];
+                                        const int klt = q[2] * (q[2] - 1) / 2 + q[3];
+                                        if (ijt > nij || (ijt == nij && klt >= nkl)) {
+                                            skip = its(its(q[0], q[1]), its(q[2], q[3])) != 1;
+                                            continue;
+                                        }
+                                        if (ijkls > nsyt)
+                                            return fail(ctx, CIPSI_ERR_UNSUPPORTED, "sym: directory beyond nsyt entries");
+                                        const int ijklt = ijt * (ijt - 1) / 2 + klt;
+                                        const int kt_t = h_kt[ijklt - 1];
+                                        if (kt_t == 1 || kt_t == 10) perm = perm == 5 ? 10 : 1;
+                                        h_kt[ijkls - 1] = block_type(perm, kt_t);
+                                        h_nad[ijkls - 1] = sgn * h_nad[ijklt - 1];
+                                        skip = true;
+                                    }
+                                    if (skip) continue;
                                     if (yr) {
                                         // transd fills the class quartet's block; it finds no
                                         // integral of this class quartet -> skipped
@@ -637,8 +700,8 @@ extern "C" int cipsi_ijkl_sym_transform_labels(
                                         ysaut = touched[key] == 0;
                                     }
                                     if (ysaut) continue;
-                                    const int ijkls = nij * (nij - 1) / 2 + nkl;
-                                    if (ijkls > nsyt) return fail(ctx, CIPSI_ERR_UNSUPPORTED, "sym: directory overflow");
+                                    if (ijkls > nsyt)
+                                        return fail(ctx, CIPSI_ERR_UNSUPPORTED, "sym: directory beyond nsyt entries");
                                     HostBlock B{};
                                     B.ii = ii; B.jj = jj; B.kk = kk; B.ll = ll;
                                     B.nij = nij; B.nkl = nkl; B.ijkls = ijkls;
@@ -660,7 +723,9 @@ extern "C" int cipsi_ijkl_sym_transform_labels(
         B.h_off = hsz;
         hsz += even(B.npq * B.nlk);
         B.out_off = total;
-        B.out_cnt = B.yijkl ? npr(B.nlk) : B.nlk * B.nijc;
+        // a triangular block: row ilk holds min(ilk, nijc) values (tijkld.f:75-77)
+        B.out_cnt = !B.yijkl ? B.nlk * B.nijc
+                  : B.nijc >= B.nlk ? npr(B.nlk) : npr(B.nijc) + (B.nlk - B.nijc) * B.nijc;
         total += B.out_cnt;
     }
     *mo_count = total;

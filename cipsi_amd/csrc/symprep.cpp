@@ -4,6 +4,10 @@
 //   group tables   otodon.f:163-317 with the tables of BLOCKDATA1.f: irreps, their
 //                  product table and characters for the non-degenerate groups
 //   symmetry AOs   otodon.f:320-353: per irrep, which AOs carry its symmetry orbitals
+//   linear groups  otodon.f:36-144: C-inf-v and D-inf-h, whose AOs ijkl classifies by
+//                  their angular character about the axis (and g / u for D-inf-h), with
+//                  the product table and the two symmetry transformations (rotations by
+//                  90 and 45 degrees) that itijkl uses to skip equivalent blocks
 //   rotat2         rotat2.f:50-423: frozen-column compaction; the symmetry vectors
 //                  normalised with S; classes of irreps with the same AOs, made contiguous;
 //                  levels of degenerate MOs (the MO diagonal of h, tmono.f:37-54); the
@@ -15,8 +19,10 @@
 //
 // Host FP64 code built without fused multiply-add, summing in the reference's order, so
 // the coefficients equal the reference's bit for bit (its ijom file).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -79,59 +85,16 @@ extern "C" int cipsi_ijkl_sym_group(const char* group, int naxis, int nt, int* n
     return CIPSI_OK;
 }
 
-extern "C" int cipsi_ijkl_sym_prepare(int n_ao, int nsym, const int* its, const int* jsym, int nt,
-                                      int nshell, const int* ktype, const int* kloc,
-                                      const int* newsh, const int* iptr, const int* idtr,
-                                      const int* iftr, const double* s_packed,
-                                      const double* h_packed, const double* c_square, int nvecf,
-                                      const int* ygel, int noc, double trec, double tdeg,
-                                      double* c_out, int* itsyv, int* isymp, int* nsymp,
-                                      int* ivecsy, int* nbp, int* nbo, int* ind, double* c_packed,
-                                      int* tag) {
-    (void)its;
-    if (n_ao < 1 || nsym < 2 || nsym > 8 || nt < 1 || nshell < 1 || nvecf < 1 || nvecf > n_ao ||
-        noc < 0 || !jsym || !ktype || !kloc || !newsh || !iptr || !idtr || !iftr || !h_packed ||
-        !c_square || !c_out || !itsyv || !isymp || !nsymp || !ivecsy || !nbp || !nbo || !ind ||
-        !c_packed)
-        return CIPSI_ERR_ARG;
-    const int n = n_ao;
+namespace {
+
+// rotat2.f:50-423 and vector.f:17-46 for the symmetry-orbital table vs (nsym x n, the
+// signed orbital number of each AO per irrep, as otodon leaves it in ivecsy).
+int rotat2_vector(int n, int nsym, std::vector<int>& vs, const double* s_packed,
+                  const double* h_packed, const double* c_square, int nvecf, const int* ygel,
+                  int noc, double trec, double tdeg, double* c_out, int* itsyv, int* isymp,
+                  int* nsymp, int* ivecsy, int* nbp, int* nbo, int* ind, double* c_packed,
+                  int* tag) {
     const int64_t nn = (int64_t)n * n;
-    if (tag) *tag = 0;
-    // ---- otodon.f:320-353: per irrep, AOs of its symmetry orbitals (signed SO number)
-    std::vector<int> vs((size_t)nsym * n, 0);
-    {
-        int flat = 0;
-        for (int ir = 0; ir < nsym; ++ir) {
-            int so = 0;
-            for (int sh = 0; sh < nshell; ++sh) {
-                const int kt = ktype[sh];
-                if (kt < 1 || kt > 4) return CIPSI_ERR_ARG;
-                for (int comp = 0; comp < 2 * kt - 1; ++comp, ++flat) {
-                    if (flat >= nsym * n) return CIPSI_ERR_ARG;
-                    if (vs[flat] != 0) continue;
-                    ++so;
-                    for (int g = 0; g < nt; ++g) {
-                        const int img_sh = newsh[sh + g * nshell] - 1;
-                        if (img_sh < 0 || img_sh >= nshell) return CIPSI_ERR_ARG;
-                        const int img = ir * n + kloc[img_sh] - 1 + comp;
-                        if (img < ir * n || img >= (ir + 1) * n) return CIPSI_ERR_ARG;
-                        int sgn = jsym[ir + g * nsym];
-                        if (kt == 2) sgn *= iptr[comp + 3 * g];
-                        else if (kt == 3) sgn *= idtr[comp + 6 * g];
-                        else if (kt == 4) sgn *= iftr[comp + 10 * g];
-                        const int v = so * sgn;
-                        if (vs[img] != 0 && vs[img] != v) {       // the projection vanishes
-                            for (int h = 0; h < nt; ++h)
-                                vs[ir * n + kloc[newsh[sh + h * nshell] - 1] - 1 + comp] = 0;
-                            --so;
-                            break;
-                        }
-                        vs[img] = v;
-                    }
-                }
-            }
-        }
-    }
     // ---- rotat2.f:50-60: active columns to the front (later columns stay as they were)
     std::vector<double> c(c_square, c_square + nn);
     {
@@ -325,4 +288,185 @@ extern "C" int cipsi_ijkl_sym_prepare(int n_ao, int nsym, const int* its, const 
     std::memcpy(ivecsy, vs.data(), sizeof(int) * nsym * n);
     *nsymp = ncls;
     return CIPSI_OK;
+}
+
+}  // namespace
+
+extern "C" int cipsi_ijkl_sym_prepare(int n_ao, int nsym, const int* its, const int* jsym, int nt,
+                                      int nshell, const int* ktype, const int* kloc,
+                                      const int* newsh, const int* iptr, const int* idtr,
+                                      const int* iftr, const double* s_packed,
+                                      const double* h_packed, const double* c_square, int nvecf,
+                                      const int* ygel, int noc, double trec, double tdeg,
+                                      double* c_out, int* itsyv, int* isymp, int* nsymp,
+                                      int* ivecsy, int* nbp, int* nbo, int* ind, double* c_packed,
+                                      int* tag) {
+    (void)its;
+    if (n_ao < 1 || nsym < 2 || nsym > 8 || nt < 1 || nshell < 1 || nvecf < 1 || nvecf > n_ao ||
+        noc < 0 || !jsym || !ktype || !kloc || !newsh || !iptr || !idtr || !iftr || !h_packed ||
+        !c_square || !c_out || !itsyv || !isymp || !nsymp || !ivecsy || !nbp || !nbo || !ind ||
+        !c_packed)
+        return CIPSI_ERR_ARG;
+    const int n = n_ao;
+    if (tag) *tag = 0;
+    // ---- otodon.f:320-353: per irrep, AOs of its symmetry orbitals (signed SO number)
+    std::vector<int> vs((size_t)nsym * n, 0);
+    {
+        int flat = 0;
+        for (int ir = 0; ir < nsym; ++ir) {
+            int so = 0;
+            for (int sh = 0; sh < nshell; ++sh) {
+                const int kt = ktype[sh];
+                if (kt < 1 || kt > 4) return CIPSI_ERR_ARG;
+                for (int comp = 0; comp < 2 * kt - 1; ++comp, ++flat) {
+                    if (flat >= nsym * n) return CIPSI_ERR_ARG;
+                    if (vs[flat] != 0) continue;
+                    ++so;
+                    for (int g = 0; g < nt; ++g) {
+                        const int img_sh = newsh[sh + g * nshell] - 1;
+                        if (img_sh < 0 || img_sh >= nshell) return CIPSI_ERR_ARG;
+                        const int img = ir * n + kloc[img_sh] - 1 + comp;
+                        if (img < ir * n || img >= (ir + 1) * n) return CIPSI_ERR_ARG;
+                        int sgn = jsym[ir + g * nsym];
+                        if (kt == 2) sgn *= iptr[comp + 3 * g];
+                        else if (kt == 3) sgn *= idtr[comp + 6 * g];
+                        else if (kt == 4) sgn *= iftr[comp + 10 * g];
+                        const int v = so * sgn;
+                        if (vs[img] != 0 && vs[img] != v) {       // the projection vanishes
+                            for (int h = 0; h < nt; ++h)
+                                vs[ir * n + kloc[newsh[sh + h * nshell] - 1] - 1 + comp] = 0;
+                            --so;
+                            break;
+                        }
+                        vs[img] = v;
+                    }
+                }
+            }
+        }
+    }
+    return rotat2_vector(n, nsym, vs, s_packed, h_packed, c_square, nvecf, ygel, noc, trec, tdeg,
+                         c_out, itsyv, isymp, nsymp, ivecsy, nbp, nbo, ind, c_packed, tag);
+}
+
+// ------------------------------------------------------------------ linear groups
+namespace {
+
+// The classes ijkl gives the AOs of a linear molecule, by their angular character about
+// the axis (otodon.f:37-39,62-144): sigma, pi_x, pi_y, delta_x, delta_y, phi_x, phi_y.
+// kLinComp[class][ktype - 1]: the component of a shell of that type in the class (pure
+// components, pshf's order: p = x, y, z; d = z^2, xz, yz, x^2 - y^2, xy), -1 if none.
+const int kLinComp[7][4] = {{0, 2, 0, 0},   {-1, 0, 1, 1},   {-1, 1, 2, 2},  {-1, -1, 3, 3},
+                            {-1, -1, 4, 4}, {-1, -1, -1, 5}, {-1, -1, -1, 6}};
+// Parity of each class under x -> -x (bit 0) and y -> -y (bit 1): the irrep of the C2v
+// subgroup the class reduces to.  The product table its is the subgroup's, with the
+// labels the reference's tables itscv / itsdh give the subgroup irreps (BLOCKDATA1.f):
+// C-inf-v by parity bits; D-inf-h by parity bits and g / u.
+const int kLinBits[7] = {0, 1, 2, 0, 3, 1, 2};
+const int kLinLabelCv[4] = {1, 2, 3, 5};
+const int kLinLabelDh[4][2] = {{1, 2}, {3, 4}, {5, 6}, {9, 10}};
+// The two symmetry transformations on the classes (isydec / isyded, ijkl.f:141-149):
+// the rotation by 90 degrees about the axis (pi_x -> pi_y -> -pi_x, delta -> -delta,
+// phi_x -> phi_y -> -phi_x) and the rotation by 45 degrees, kept on the sigma and delta
+// classes only (delta_x -> delta_y -> -delta_x); 0: the class has no image.
+const int kLinRot[2][7] = {{1, 3, -2, -4, -5, 7, -6}, {1, 0, 0, 5, -4, 0, 0}};
+
+}  // namespace
+
+extern "C" int cipsi_ijkl_sym_linear(const char* group, int n_ao, int nshell, const int* ktype,
+                                     const int* kloc, const double* zsh, const int* newsh, int nt,
+                                     int* nsym, int* its, int* ntrans, int* isydeg, int* so) {
+    if (!group || n_ao < 1 || nshell < 1 || !ktype || !kloc || !zsh || !nsym || !its || !ntrans ||
+        !isydeg || !so)
+        return CIPSI_ERR_ARG;
+    const bool dinfh = same_name(group, "DINFH");
+    if (!dinfh && !same_name(group, "CINFV")) return CIPSI_ERR_ARG;
+    if (dinfh && (!newsh || nt < 2)) return CIPSI_ERR_ARG;   // the image under sigma_h
+    const int n = n_ao;
+    bool has_p = false, has_d = false, has_f = false;
+    for (int sh = 0; sh < nshell; ++sh) {
+        const int kt = ktype[sh];
+        if (kt < 1 || kt > 4 || kloc[sh] < 1 || kloc[sh] - 1 + 2 * kt - 1 > n) return CIPSI_ERR_ARG;
+        if (dinfh && (newsh[sh + nshell] < 1 || newsh[sh + nshell] > nshell ||
+                      ktype[newsh[sh + nshell] - 1] != kt))
+            return CIPSI_ERR_ARG;
+        has_p = has_p || kt == 2;
+        has_d = has_d || kt == 3;
+        has_f = has_f || kt == 4;
+    }
+    // otodon.f:47-60: one transformation for the p shells, one more for the d shells (the
+    // presence of a higher shell implies the lower ones)
+    if (has_f) has_d = true;
+    if (has_d) has_p = true;
+    const int ntr = (has_p ? 1 : 0) + (has_d ? 1 : 0);
+    const int ns = dinfh ? (has_f ? 14 : 10) : (has_f ? 7 : 5);
+    std::fill(so, so + (size_t)ns * n, 0);
+    for (int ir = 0; ir < ns; ++ir) {
+        const int cls = dinfh ? ir / 2 : ir;
+        const bool gerade = dinfh && ir % 2 == 0;
+        int* row = so + (size_t)ir * n;
+        int count = 0;
+        for (int sh = 0; sh < nshell; ++sh) {
+            const int comp = kLinComp[cls][ktype[sh] - 1];
+            if (comp < 0) continue;
+            const int a = kloc[sh] - 1 + comp;
+            if (!dinfh) {                                  // one AO per shell and class
+                if (row[a] == 0) row[a] = ++count;
+                continue;
+            }
+            // D-inf-h: the component plus or minus its image on the other atom; + when the
+            // component's parity (s, d even; p, f odd) is the class's; an AO at the centre
+            // enters only the classes of its own parity
+            const bool odd = ktype[sh] % 2 == 0;
+            const bool same = odd != gerade;
+            const double z = std::fabs(zsh[sh]);
+            if (!same && z < 1.e-8) continue;
+            if (row[a] != 0) continue;
+            row[a] = ++count;
+            if (z > 1.e-8) {
+                const int b = kloc[newsh[sh + nshell] - 1] - 1 + comp;
+                if (row[b] == 0) row[b] = same ? count : -count;
+            }
+        }
+    }
+    for (int i = 0; i < ns; ++i)
+        for (int j = 0; j < ns; ++j) {
+            int v;
+            if (dinfh) {
+                const int b = kLinBits[i / 2] ^ kLinBits[j / 2], par = (i ^ j) & 1;
+                v = kLinLabelDh[b][par];
+            } else {
+                v = kLinLabelCv[kLinBits[i] ^ kLinBits[j]];
+            }
+            its[i + j * ns] = v;
+        }
+    for (int t = 0; t < ntr; ++t)
+        for (int i = 0; i < ns; ++i) {
+            int v;
+            if (dinfh) {
+                const int m = kLinRot[t][i / 2];
+                v = m == 0 ? 0 : (m > 0 ? 1 : -1) * (2 * (std::abs(m) - 1) + i % 2 + 1);
+            } else {
+                v = kLinRot[t][i];
+            }
+            isydeg[i + t * ns] = v;
+        }
+    *nsym = ns;
+    *ntrans = ntr;
+    return CIPSI_OK;
+}
+
+extern "C" int cipsi_ijkl_sym_prepare_so(int n_ao, int nsym, const int* so, const double* s_packed,
+                                         const double* h_packed, const double* c_square, int nvecf,
+                                         const int* ygel, int noc, double trec, double tdeg,
+                                         double* c_out, int* itsyv, int* isymp, int* nsymp,
+                                         int* ivecsy, int* nbp, int* nbo, int* ind,
+                                         double* c_packed, int* tag) {
+    if (n_ao < 1 || nsym < 2 || nsym > 14 || nvecf < 1 || nvecf > n_ao || noc < 0 || !so ||
+        !h_packed || !c_square || !c_out || !itsyv || !isymp || !nsymp || !ivecsy || !nbp ||
+        !nbo || !ind || !c_packed)
+        return CIPSI_ERR_ARG;
+    if (tag) *tag = 0;
+    std::vector<int> vs(so, so + (size_t)nsym * n_ao);
+    return rotat2_vector(n_ao, nsym, vs, s_packed, h_packed, c_square, nvecf, ygel, noc, trec, tdeg,
+                         c_out, itsyv, isymp, nsymp, ivecsy, nbp, nbo, ind, c_packed, tag);
 }

@@ -24,6 +24,8 @@ module cipsi_ijkl_mod
     integer(c_int) :: n_ao, nsym, nt, nshell
     type(c_ptr) :: its, isymp, nbp, nbo, ind, ivecsy, c
     type(c_ptr) :: ishell, kloc, ktyp, numc, newsh, iptr, idtr, iftr
+    integer(c_int) :: ntrsy = 0
+    type(c_ptr) :: isydeg = c_null_ptr
   end type
 
   interface
@@ -142,6 +144,29 @@ module cipsi_ijkl_mod
       integer(c_int), value :: n_ao, nsym, nt, nshell, nvecf, noc
       integer(c_int), intent(in) :: its(*), jsym(*), ktype(*), kloc(*), newsh(*), iptr(*), &
                                     idtr(*), iftr(*), ygel(*)
+      real(c_double), intent(in) :: s_packed(*), h_packed(*), c_square(*)
+      real(c_double), value :: trec, tdeg
+      real(c_double), intent(out) :: c_out(*), c_packed(*)
+      integer(c_int), intent(out) :: itsyv(*), isymp(*), nsymp, ivecsy(*), nbp(*), nbo(*), &
+                                     ind(*), tag
+    end function
+    ! The linear groups CINFV / DINFH: otodon.f:36-144 (tables and symmetry orbitals from
+    ! the basis), then rotat2 / vector on otodon's table.
+    integer(c_int) function cipsi_ijkl_sym_linear(group, n_ao, nshell, ktype, kloc, zsh, &
+        newsh, nt, nsym, its, ntrans, isydeg, so) bind(C, name="cipsi_ijkl_sym_linear")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: group(*)
+      integer(c_int), value :: n_ao, nshell, nt
+      integer(c_int), intent(in) :: ktype(*), kloc(*), newsh(*)
+      real(c_double), intent(in) :: zsh(*)
+      integer(c_int), intent(out) :: nsym, its(*), ntrans, isydeg(*), so(*)
+    end function
+    integer(c_int) function cipsi_ijkl_sym_prepare_so(n_ao, nsym, so, s_packed, h_packed, &
+        c_square, nvecf, ygel, noc, trec, tdeg, c_out, itsyv, isymp, nsymp, ivecsy, nbp, nbo, &
+        ind, c_packed, tag) bind(C, name="cipsi_ijkl_sym_prepare_so")
+      import :: c_int, c_double
+      integer(c_int), value :: n_ao, nsym, nvecf, noc
+      integer(c_int), intent(in) :: so(*), ygel(*)
       real(c_double), intent(in) :: s_packed(*), h_packed(*), c_square(*)
       real(c_double), value :: trec, tdeg
       real(c_double), intent(out) :: c_out(*), c_packed(*)

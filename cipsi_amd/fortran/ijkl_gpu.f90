@@ -1,6 +1,7 @@
 ! ijkl_gpu -- drop-in host program for the ijkl step, running the AO->MO transform on
-! an MI355X through the C ABI of libcipsi_ijkl.so.  C1 and the point groups without
-! degenerate irreps (CS, CI, CN, CNV, CNH, DN, DNH with a two-fold axis).
+! an MI355X through the C ABI of libcipsi_ijkl.so.  C1, the point groups without
+! degenerate irreps (CS, CI, CN, CNV, CNH, DN, DNH with a two-fold axis) and the linear
+! groups CINFV and DINFH with their symmetry transformations.
 !
 ! It keeps ijkl's file contract so that fock and the rest of the chain run unchanged:
 !   stdin   &ijfil (file names, CIPSI_sources/ijkl/openf.f:6-34) then &ijklin
@@ -15,8 +16,10 @@
 !   unit 50 f50:   packed MO integrals in 8192-double records, the last written full
 !                  length (tijkld.f:68-88), then the h_MO record (tmono.f:56)
 ! With symmetry the MO preparation of otodon / rotat2 / vector runs in the library
-! (cipsi_ijkl_sym_prepare) and the f50 stream holds the symmetry-allowed blocks in the
-! order of itijkl.f:70-247, with their directory in f25.  The unit-40 scratch file of
+! (cipsi_ijkl_sym_prepare; cipsi_ijkl_sym_linear and cipsi_ijkl_sym_prepare_so for the
+! linear groups) and the f50 stream holds the symmetry-allowed blocks in the order of
+! itijkl.f:70-247, less those the symmetry transformations make redundant, with their
+! directory in f25.  The unit-40 scratch file of
 ! the reference is not needed: the intermediate stays in device memory.  Errors are
 ! reported like ijkl's (' !! ERROR in ijkl', a message and an i2 tag read by
 ! Autocip_13/calcul.f90:169-176): rotat2's own tags (2, 1, -1) where it would stop,
@@ -119,7 +122,6 @@ program ijkl_gpu
   if (norb > doa) call fail(' le nombre d''orbitales depasse la limite du programme', 9)
   if (noc == 0) noc = (ne + 1) / 2
   if (nvecf == 0) nvecf = nveca
-  if (ntrans == -1) ntrans = 0
   nnorb = norb * norb
   ntri = norb * (norb + 1) / 2
   if (.not. (nt == 1 .and. (grouph(1:2) == 'C1' .or. grouph(1:2) == 'c1'))) then
@@ -127,6 +129,7 @@ program ijkl_gpu
     stop
   end if
 
+  if (ntrans == -1) ntrans = 0
   ! ------------------------------------------------------------------ MO coefficients
   ! rotat2 for C1 (rotat2.f:33-60, 71, 386-420): read record 7, compact the active
   ! (not frozen) columns to the front; the symmetry classification and the
@@ -219,15 +222,18 @@ contains
   ! The ijkl step for a point group with nsym > 1 (ijkl.f:186-430 with otodon, rotat2,
   ! vector and itijkl in the library).
   subroutine run_symmetric()
-    integer(c_int), target :: its(64), jsym(64), isymp_t(nsymz), nbp(nsymz), nbo(nsymz), &
-                              ind(nsymz + 1)
+    integer(c_int), target :: its(196), jsym(64), isymp_t(nsymz), nbp(nsymz), nbo(nsymz), &
+                              ind(nsymz + 1), isydeg_l(28)
     integer(c_int), allocatable, target :: ktype(:), kloc(:), newsh_t(:), iptr(:), idtr(:), &
                                            iftr(:), ishell(:), ktyp(:), numc(:), ivecsy_t(:), &
-                                           itsyv_t(:), ygel_t(:), ipair(:)
+                                           itsyv_t(:), ygel_t(:), ipair(:), so(:), &
+                                           isydeg_t(:)
+    double precision, allocatable :: zsh(:)
     integer(c_int16_t), allocatable :: nad(:), ktypsy(:)
     double precision, allocatable, target :: c_out(:), c_packed(:), stream(:)
     character(kind=c_char) :: gname(9)
-    integer(c_int) :: nsym_g, nsymp, tag, npaire, max_pairs
+    integer(c_int) :: nsym_g, nsymp, tag, npaire, max_pairs, ntran
+    logical :: linear, dinfh
     integer(c_int64_t) :: count
     type(cipsi_ijkl_sym_t) :: sym
     integer :: sh, j, g, nf, a, nsyt
@@ -236,16 +242,23 @@ contains
     if (nveci /= 1) call fail(' nveci must be 1', 9)
     if (yuhf) call fail(' UHF orbitals (two passes) are not supported', 9)
     if (itsyv(1) /= 0) call fail(' MO symmetries given in the namelist are not supported', 9)
-    ! ---- group tables (otodon.f:163-317)
+    if (.not. yauto) call fail(' only yauto=T (tables from the group name) is supported', 9)
     do j = 1, 8
       gname(j) = grouph(j:j)
+      if (gname(j) >= 'a' .and. gname(j) <= 'z') gname(j) = achar(iachar(gname(j)) - 32)
     end do
     gname(9) = c_null_char
-    rc = cipsi_ijkl_sym_group(gname, int(naxh, c_int), int(nt, c_int), nsym_g, its, jsym)
-    if (rc == CIPSI_ERR_UNSUPPORTED) &
-      call fail(' groupe '//trim(grouph)//': only groups without degenerate irreps', 9)
-    if (rc /= CIPSI_OK) call fail(' groupe non trouve: '//trim(grouph), 9)
-    nsym = nsym_g
+    dinfh = grouph(1:5) == 'DINFH' .or. grouph(1:5) == 'dinfh'
+    linear = dinfh .or. grouph(1:5) == 'CINFV' .or. grouph(1:5) == 'cinfv'
+    ntran = 0
+    if (.not. linear) then
+      ! ---- group tables (otodon.f:163-317)
+      rc = cipsi_ijkl_sym_group(gname, int(naxh, c_int), int(nt, c_int), nsym_g, its, jsym)
+      if (rc == CIPSI_ERR_UNSUPPORTED) &
+        call fail(' programme stoppe : groupe degenere', 9)
+      if (rc /= CIPSI_OK) call fail(' groupe non trouve: '//trim(grouph), 9)
+      nsym = nsym_g
+    end if
     ! ---- basis tables of rondof.f:40-118 (pshf: five pure d, seven pure f functions)
     allocate(ktype(nshell), kloc(nshell), newsh_t(nshell * nt), iptr(3 * nt), idtr(6 * nt), &
              iftr(10 * nt), ishell(norb), ktyp(norb), numc(norb))
@@ -281,6 +294,27 @@ contains
       end do
     end do
     if (a /= norb) call fail(' basis tables and N_AO disagree', 9)
+    if (linear) then
+      ! ---- otodon.f:36-144: classes by angular character about the axis
+      allocate(zsh(nshell), so(14 * norb))
+      do sh = 1, nshell
+        k = ibas(5 * (sh - 1) + 4)                 ! katom
+        if (k < 1 .or. k > nat) call fail(' basis tables and atoms disagree', 9)
+        zsh(sh) = atom_c(3 * k)
+      end do
+      rc = cipsi_ijkl_sym_linear(gname, int(norb, c_int), int(nshell, c_int), ktype, kloc, &
+                                 zsh, newsh_t, int(nt, c_int), nsym_g, its, ntran, isydeg_l, so)
+      if (rc /= CIPSI_OK) call fail(' cipsi_ijkl_sym_linear: '//cipsi_ijkl_strerror(rc), 9)
+      nsym = nsym_g
+    end if
+    ! ijkl.f:105,141-149: ntrans from otodon unless the namelist sets it; one symmetry
+    ! transformation per ntrans (yauto), columns past otodon's are zero
+    if (ntrans == -1) ntrans = ntran
+    allocate(isydeg_t(nsym * max(ntrans, 1)))
+    isydeg_t = 0
+    do j = 1, min(ntrans, ntran)
+      isydeg_t((j - 1) * nsym + 1:j * nsym) = isydeg_l((j - 1) * nsym + 1:j * nsym)
+    end do
     ! ---- records 2, 3, 7
     allocate(c(norb, norb), h(ntri), s(ntri))
     read(2) s
@@ -301,11 +335,17 @@ contains
       ygel_t(j) = 0
       if (ygel(j)) ygel_t(j) = 1
     end do
-    rc = cipsi_ijkl_sym_prepare(int(norb, c_int), nsym, its, jsym, int(nt, c_int), &
-                                int(nshell, c_int), ktype, kloc, newsh_t, iptr, idtr, iftr, s, &
-                                h, c, int(nvecf, c_int), ygel_t, int(noc, c_int), trec, tdeg, &
-                                c_out, itsyv_t, isymp_t, nsymp, ivecsy_t, nbp, nbo, ind, &
-                                c_packed, tag)
+    if (linear) then
+      rc = cipsi_ijkl_sym_prepare_so(int(norb, c_int), nsym, so, s, h, c, int(nvecf, c_int), &
+                                     ygel_t, int(noc, c_int), trec, tdeg, c_out, itsyv_t, &
+                                     isymp_t, nsymp, ivecsy_t, nbp, nbo, ind, c_packed, tag)
+    else
+      rc = cipsi_ijkl_sym_prepare(int(norb, c_int), nsym, its, jsym, int(nt, c_int), &
+                                  int(nshell, c_int), ktype, kloc, newsh_t, iptr, idtr, iftr, &
+                                  s, h, c, int(nvecf, c_int), ygel_t, int(noc, c_int), trec, &
+                                  tdeg, c_out, itsyv_t, isymp_t, nsymp, ivecsy_t, nbp, nbo, ind, &
+                                  c_packed, tag)
+    end if
     if (rc == CIPSI_ERR_NOT_ORTHONORMAL) then
       write(6, '(a)') ' rotat2: the MOs do not separate by symmetry or are not orthonormal'
       call fail(' fix ?, error tag =', int(tag))
@@ -327,6 +367,11 @@ contains
     if (istop == 1) stop
     write(25) nsym, nvec, noc, ntrans, (itsyv(j), j = 1, nvec), norb, ydp, chstan
     write(25) ((its(a + (j - 1) * nsym), a = 1, nsym), j = 1, nsym)
+    if (ntrans /= 0 .and. yhondo) then            ! ijkl.f:254-261
+      do j = 1, ntrans
+        write(25) 1, (isydeg_t(a + (j - 1) * nsym), a = 1, nsym)
+      end do
+    end if
     ! ---- pqrs stream and the transform (itijkl.f:57-298)
     n_entries = int(nrec, c_int64_t) * lsize
     allocate(xx(max(n_entries, 1_c_int64_t)), ix(max(n_entries, 1_c_int64_t)))
@@ -357,6 +402,8 @@ contains
     sym%iptr = c_loc(iptr)
     sym%idtr = c_loc(idtr)
     sym%iftr = c_loc(iftr)
+    sym%ntrsy = int(ntrans, c_int)
+    sym%isydeg = c_loc(isydeg_t)
     rc = cipsi_ijkl_create(ctx, 0_c_int)
     if (rc /= CIPSI_OK) call fail(' cipsi_ijkl_create: '//cipsi_ijkl_strerror(rc), 9)
     rc = cipsi_ijkl_sym_transform_labels(ctx, sym, xx, ix, n_entries, c_null_ptr, 0_c_int64_t, &

@@ -51,7 +51,8 @@ EXPORTS = [
     "cipsi_ijkl_batch_class_info", "cipsi_ijkl_batch_set_class_plan",
     "cipsi_ijkl_gather_dev", "cipsi_ijkl_fock", "cipsi_ijkl_fock_dev",
     "cipsi_ijkl_hdig_dev", "cipsi_ijkl_hntd_dev", "cipsi_ijkl_sym_transform_labels",
-    "cipsi_ijkl_sym_group", "cipsi_ijkl_sym_prepare",
+    "cipsi_ijkl_sym_group", "cipsi_ijkl_sym_prepare", "cipsi_ijkl_sym_linear",
+    "cipsi_ijkl_sym_prepare_so",
 ]
 
 
@@ -63,7 +64,8 @@ class SymDesc(ctypes.Structure):
                                                             "ivecsy")] + \
                [("c", ctypes.POINTER(ctypes.c_double))] + \
                [(f, ctypes.POINTER(ctypes.c_int)) for f in ("ishell", "kloc", "ktyp", "numc",
-                                                            "newsh", "iptr", "idtr", "iftr")]
+                                                            "newsh", "iptr", "idtr", "iftr")] + \
+               [("ntrsy", ctypes.c_int), ("isydeg", ctypes.POINTER(ctypes.c_int))]
 
 
 class IjklError(RuntimeError):
@@ -154,6 +156,10 @@ def load_library(path: str = LIB_PATH):
     L.cipsi_ijkl_sym_group.argtypes = [ctypes.c_char_p, _c_int, _c_int, _ipi, _ipi, _ipi]
     L.cipsi_ijkl_sym_prepare.argtypes = [_c_int, _c_int, _vp, _vp, _c_int, _c_int] + [_vp] * 6 + \
         [_vp, _vp, _vp, _c_int, _vp, _c_int, ctypes.c_double, ctypes.c_double] + [_vp] * 9 + [_vp]
+    L.cipsi_ijkl_sym_linear.argtypes = [ctypes.c_char_p, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_int,
+                                        _ipi, _vp, _ipi, _vp, _vp]
+    L.cipsi_ijkl_sym_prepare_so.argtypes = [_c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int,
+                                            ctypes.c_double, ctypes.c_double] + [_vp] * 9 + [_vp]
     _i16p = ctypes.POINTER(ctypes.c_int16)
     L.cipsi_ijkl_sym_transform_labels.argtypes = [_vp, ctypes.POINTER(SymDesc), _vp, _vp, _c_i64,
                                                   _vp, _c_i64, ctypes.POINTER(_c_i64), _vp, _c_int,
@@ -217,6 +223,69 @@ def sym_prepare(tables, s_packed, h_packed, c_square, noc, nvecf=None, ygel=None
                                   ctypes.byref(tag))
     if rc != 0:
         err = IjklError(rc, "cipsi_ijkl_sym_prepare", f"rotat2 error tag {tag.value}")
+        err.tag = tag.value
+        raise err
+    return dict(c=c_out, itsyv=itsyv, isymp=isymp, nsymp=nsymp.value,
+                ivecsy=ivecsy.reshape(nsym, n), nbp=nbp, nbo=nbo, ind=ind,
+                c_packed=cpk[:int(ind[nsym]) - 1].copy())
+
+
+def sym_linear(group: str, tables):
+    """otodon's tables for the linear groups CINFV / DINFH (cipsi_ijkl_sym_linear,
+    otodon.f:36-144).  tables: norb, ktype, kloc (per shell, 1-based), zsh (z of each
+    shell's atom), newsh (nshell x nt, DINFH).  Returns (nsym, its (nsym x nsym), ntrans,
+    isydeg (nsym x ntrans), so (nsym x norb))."""
+    L = load_library()
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    n = int(tables["norb"])
+    ktype, kloc = i32(tables["ktype"]), i32(tables["kloc"])
+    zsh = np.ascontiguousarray(tables["zsh"], dtype=np.float64)
+    newsh = i32(np.asarray(tables["newsh"]).T.ravel())
+    nt = int(np.asarray(tables["newsh"]).shape[1])
+    ns, ntr = _c_int(0), _c_int(0)
+    its = np.zeros(14 * 14, dtype=np.int32)
+    isydeg = np.zeros(14 * 2, dtype=np.int32)
+    so = np.zeros(14 * n, dtype=np.int32)
+    rc = L.cipsi_ijkl_sym_linear(group.encode().ljust(8), n, ktype.size, _ptr(ktype), _ptr(kloc),
+                                 _ptr(zsh), _ptr(newsh), nt, ctypes.byref(ns), _ptr(its),
+                                 ctypes.byref(ntr), _ptr(isydeg), _ptr(so))
+    if rc != 0:
+        raise IjklError(rc, "cipsi_ijkl_sym_linear", L.cipsi_ijkl_strerror(rc).decode())
+    k, t = ns.value, ntr.value
+    return (k, its[:k * k].reshape(k, k, order="F").copy(), t,
+            isydeg[:k * t].reshape(k, t, order="F").copy(), so[:k * n].reshape(k, n).copy())
+
+
+def sym_prepare_so(n_ao, nsym, so, s_packed, h_packed, c_square, noc, nvecf=None, ygel=None,
+                   trec=1e-5, tdeg=1e-6):
+    """rotat2 + vector for otodon's symmetry-orbital table so (nsym x n_ao), the linear
+    groups' route (cipsi_ijkl_sym_prepare_so).  Returns what sym_prepare returns."""
+    L = load_library()
+    n = int(n_ao)
+    nvecf = n if nvecf is None else nvecf
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    so = i32(np.asarray(so).ravel())
+    sp = None if s_packed is None else np.ascontiguousarray(s_packed, dtype=np.float64)
+    hp = np.ascontiguousarray(h_packed, dtype=np.float64)
+    cs = np.asfortranarray(c_square, dtype=np.float64)
+    yg = None if ygel is None else i32(ygel)
+    c_out = np.zeros((n, n), order="F")
+    itsyv = np.zeros(nvecf, dtype=np.int32)
+    isymp = np.zeros(nsym, dtype=np.int32)
+    nsymp = _c_int(0)
+    ivecsy = np.zeros(nsym * n, dtype=np.int32)
+    nbp = np.zeros(nsym, dtype=np.int32)
+    nbo = np.zeros(nsym, dtype=np.int32)
+    ind = np.zeros(nsym + 1, dtype=np.int32)
+    cpk = np.zeros(n * n)
+    tag = _c_int(0)
+    pv = lambda a: None if a is None else _ptr(a)
+    rc = L.cipsi_ijkl_sym_prepare_so(n, nsym, _ptr(so), pv(sp), _ptr(hp), _ptr(cs), nvecf, pv(yg),
+                                     noc, trec, tdeg, _ptr(c_out), _ptr(itsyv), _ptr(isymp),
+                                     ctypes.byref(nsymp), _ptr(ivecsy), _ptr(nbp), _ptr(nbo),
+                                     _ptr(ind), _ptr(cpk), ctypes.byref(tag))
+    if rc != 0:
+        err = IjklError(rc, "cipsi_ijkl_sym_prepare_so", f"rotat2 error tag {tag.value}")
         err.tag = tag.value
         raise err
     return dict(c=c_out, itsyv=itsyv, isymp=isymp, nsymp=nsymp.value,
@@ -349,7 +418,8 @@ class Context:
         tables: dict of the COMMON state after vector.f (1-based, as the reference holds
         it): n_ao, nsym, nt, its (nsym x nsym), isymp, nbp, nbo, ind, ivecsy (nsym x n_ao),
         c (vector.f's packed coefficients), ishell, ktyp, numc (per AO), kloc (per shell),
-        newsh (nshell x nt), iptr (3 x nt), idtr (6 x nt), iftr (10 x nt).  xx, ix: the
+        newsh (nshell x nt), iptr (3 x nt), idtr (6 x nt), iftr (10 x nt), and for the
+        linear groups isydeg (nsym x ntrsy, the symmetry transformations).  xx, ix: the
         pshf skeleton label stream (numpy).  Returns (mo_stream, ipair (npaire x 2), nad,
         ktypsy) -- the f50 stream and the f25 directory of itijkl.f:208-213,291-296."""
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
@@ -360,6 +430,9 @@ class Context:
                     ishell=i32(tables["ishell"]), kloc=i32(tables["kloc"]), ktyp=i32(tables["ktyp"]),
                     numc=i32(tables["numc"]), newsh=fort(tables["newsh"]), iptr=fort(tables["iptr"]),
                     idtr=fort(tables["idtr"]), iftr=fort(tables["iftr"]))
+        isydeg = tables.get("isydeg")
+        ntrsy = 0 if isydeg is None else int(np.asarray(isydeg).shape[1])
+        keep["isydeg"] = fort(isydeg) if ntrsy else np.zeros(1, dtype=np.int32)
         cpk = np.ascontiguousarray(tables["c"], dtype=np.float64)
         ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
         nshell = np.asarray(tables["newsh"]).shape[0]
@@ -367,7 +440,8 @@ class Context:
                        *[ip(keep[k]) for k in ("its", "isymp", "nbp", "nbo", "ind", "ivecsy")],
                        cpk.ctypes.data_as(_dp),
                        *[ip(keep[k]) for k in ("ishell", "kloc", "ktyp", "numc", "newsh", "iptr",
-                                               "idtr", "iftr")])
+                                               "idtr", "iftr")],
+                       ntrsy, ip(keep["isydeg"]))
         xx = np.ascontiguousarray(xx, dtype=np.float64)
         ix = np.ascontiguousarray(ix, dtype=np.int32)
         nsym = int(tables["nsym"])

@@ -135,22 +135,45 @@ int cipsi_ijkl_transform_packed_noalloc_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n
  *   ishell, ktyp, numc   per AO: shell, shell type, component (rondof.f:77-82)
  *   kloc     per shell: first AO; newsh: nshell x nt image shells (rondof.f:30-38)
  *   iptr, idtr, iftr     3 x nt, 6 x nt, 10 x nt diagonal signs (rondof.f:100-118)
- * Groups with degenerate-irrep equivalences (C-inf-v / D-inf-h, ntrsy > 0) are not
- * handled; every AO must belong to the irreps of one class only (true for diatomics in
- * C2v and D2h). */
+ *   ntrsy, isydeg  the symmetry transformations of the linear groups (blank COMMON,
+ *            ijkl.f:141-149): nsym x ntrsy signed irrep maps, column-major; ntrsy = 0
+ *            (isydeg unused) for the other groups or with ntrans = 0 in the namelist.
+ *            itijkl skips every irrep quartet a transformation maps onto an earlier one
+ *            (itijkl.f:121-200).
+ * Every AO must belong to the irreps of one class only (true for diatomics in C2v, D2h,
+ * C-inf-v and D-inf-h; not for a linear molecule with an atom at the centre). */
 typedef struct {
     int n_ao, nsym, nt, nshell;
     const int *its, *isymp, *nbp, *nbo, *ind, *ivecsy;
     const double *c;
     const int *ishell, *kloc, *ktyp, *numc, *newsh, *iptr, *idtr, *iftr;
+    int ntrsy;
+    const int *isydeg;
 } cipsi_ijkl_sym;
 
 /* otodon.f:163-317: irreps, product table its (nsym x nsym, column-major) and characters
    jsym (nsym x nt, irrep x operation) of a point group without degeneracies, from the
    info file's group name (8 characters, blank padded, either case) and naxis: C1, CS,
-   CI, CN, CNV, CNH, DN, DNH.  CINFV / DINFH (degenerate irreps, ntrsy > 0) and an axis
-   order other than 2 give CIPSI_ERR_UNSUPPORTED; an unknown name CIPSI_ERR_ARG. */
+   CI, CN, CNV, CNH, DN, DNH.  CINFV / DINFH (their tables depend on the basis: use
+   cipsi_ijkl_sym_linear) and an axis order other than 2 give CIPSI_ERR_UNSUPPORTED; an
+   unknown name CIPSI_ERR_ARG. */
 int cipsi_ijkl_sym_group(const char *group, int naxis, int nt, int *nsym, int *its, int *jsym);
+
+/* otodon.f:36-144: the linear groups CINFV and DINFH, as pshf hands them over (the info
+   file names the linear group and carries the operations of C1, or of Cs = (E, sigma_h)
+   for DINFH, ptgrp.f).  ijkl classifies the AOs by their angular character about the
+   axis: sigma, pi_x, pi_y, delta_x, delta_y (phi_x, phi_y with f shells), times g / u for
+   DINFH, whose symmetry orbitals are an AO plus or minus its image on the other atom.
+   Inputs per shell: ktype, kloc (1-based first AO), zsh (z coordinate of the shell's
+   atom), and for DINFH newsh (nshell x nt, column-major, nt >= 2: column 2 is the image
+   under sigma_h).  Outputs: nsym (5 or 7; 10 or 14), its (nsym x nsym, column-major,
+   room for 14 x 14), ntrans (0-2: one symmetry transformation when p shells are
+   present, two with d shells) and isydeg (nsym x ntrans, column-major, room for 14 x 2),
+   so (nsym x n_ao, room for 14 n_ao): otodon's signed symmetry-orbital numbers, the
+   input of cipsi_ijkl_sym_prepare_so. */
+int cipsi_ijkl_sym_linear(const char *group, int n_ao, int nshell, const int *ktype,
+                          const int *kloc, const double *zsh, const int *newsh, int nt,
+                          int *nsym, int *its, int *ntrans, int *isydeg, int *so);
 
 /* The MO preparation ijkl does before itijkl for nsym > 1, host FP64 (no device work):
    otodon.f:320-353 (AOs of each irrep's symmetry orbitals), rotat2.f:50-423 (frozen
@@ -176,6 +199,16 @@ int cipsi_ijkl_sym_prepare(int n_ao, int nsym, const int *its, const int *jsym, 
                            int *nsymp, int *ivecsy, int *nbp, int *nbo, int *ind,
                            double *c_packed, int *tag);
 
+/* cipsi_ijkl_sym_prepare after otodon has run elsewhere: rotat2.f:50-423 and
+   vector.f:17-46 for the symmetry-orbital table so (nsym x n_ao, the signed numbers
+   otodon leaves in ivecsy), the route of the linear groups (cipsi_ijkl_sym_linear).
+   nsym <= 14; the other arguments and the outputs as cipsi_ijkl_sym_prepare. */
+int cipsi_ijkl_sym_prepare_so(int n_ao, int nsym, const int *so, const double *s_packed,
+                              const double *h_packed, const double *c_square, int nvecf,
+                              const int *ygel, int noc, double trec, double tdeg,
+                              double *c_out, int *itsyv, int *isymp, int *nsymp, int *ivecsy,
+                              int *nbp, int *nbo, int *ind, double *c_packed, int *tag);
+
 /* itijkl.f:42-298 for nsym > 1: the pshf skeleton label stream (transd.f regenerates the
    symmetry images, transd.f:125-244) -> the f50 MO stream of every symmetry-allowed
    block, in itijkl's order (class quartets, then irrep quartets), each block in
@@ -184,7 +217,12 @@ int cipsi_ijkl_sym_prepare(int n_ao, int nsym, const int *its, const int *jsym, 
    nad / ktypsy over nsyt = min(2000, P(P(nsym))) entries.  mo_count receives the
    stream length; with mo_stream == NULL only the sizes and directory are computed (the
    label stream is still unpacked: which class quartets hold integrals decides the
-   directory).  Host pointers, synchronous.  FP64 MFMA. */
+   directory).  With symmetry transformations (sym->ntrsy > 0) a block equivalent to an
+   earlier one is left out of the stream; its directory entry points at that block (nad
+   negative when the transformation changes the sign, ktypsy the index permutation,
+   itijkl.f:121-200), and a block the transformations map onto itself with its two pairs
+   exchanged is stored as a triangle (ktypsy 1).  A directory beyond nsyt entries (D-inf-h
+   with f shells) gives CIPSI_ERR_UNSUPPORTED.  Host pointers, synchronous.  FP64 MFMA. */
 int cipsi_ijkl_sym_transform_labels(cipsi_ijkl_ctx *ctx, const cipsi_ijkl_sym *sym,
                                     const double *xx, const int32_t *labels, int64_t n_entries,
                                     double *mo_stream, int64_t mo_capacity, int64_t *mo_count,

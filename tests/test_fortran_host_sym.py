@@ -1,6 +1,7 @@
 """The Fortran host (cipsi_amd/fortran/ijkl_gpu) on molecules with symmetry, without a
 GPU: istop = 1 stops after the MO preparation and the ijom file (ijkl.f:328-329), so
-otodon / rotat2 / vector of the host (through cipsi_ijkl_sym_prepare) are checked
+otodon / rotat2 / vector of the host (through cipsi_ijkl_sym_prepare, and
+cipsi_ijkl_sym_linear + cipsi_ijkl_sym_prepare_so for C-inf-v / D-inf-h) are checked
 against the reference's ijom bytes; an MO that mixes irreps stops the host with
 rotat2's own error marker and tag, as the flang-built reference does on the same files."""
 import os
@@ -12,7 +13,7 @@ import pytest
 
 from cipsi_amd import formats
 from tests.conftest import ROOT, load_golden
-from tests.test_oracle_sym import SYM, regenerate
+from tests.test_oracle_sym import LIN, SYM, regenerate
 
 HOST = os.path.join(ROOT, "cipsi_amd", "fortran", "ijkl_gpu")
 REF = os.path.join(ROOT, "oracle", "_ref", "ijkl_ref")
@@ -29,7 +30,7 @@ def run(exe, wd, stdin):
     return subprocess.run([exe], input=stdin.encode(), cwd=wd, capture_output=True, timeout=120)
 
 
-@pytest.mark.parametrize("name", SYM)
+@pytest.mark.parametrize("name", SYM + LIN)
 def test_host_prepare_only_matches_reference_ijom(name):
     g = load_golden(name)
     case = regenerate(g)

@@ -100,9 +100,12 @@ def test_fortran_host_split_precision():
 
 
 @pytest.mark.parametrize("name", ["sym_c2v_n19", "sym_c2v_deg_n19", "sym_d2h_n20", "sym_d2h_n24",
-                                  "sym_c2v_n30", "sym_d2h_n38"])
+                                  "sym_c2v_n30", "sym_d2h_n38", "lin_cinfv_n14", "lin_dinfh_n18",
+                                  "lin_cinfv_sp_n9", "lin_dinfh_sp_n10", "lin_dinfh_s_n4",
+                                  "lin_cinfv_n28", "lin_dinfh_n36"])
 def test_fortran_host_symmetry_matches_reference_files(name):
-    """Molecules with point-group symmetry (C2v, D2h): f25 (header, product table, block
+    """Molecules with point-group symmetry (C2v, D2h, and the linear C-inf-v, D-inf-h with
+    their symmetry transformations): f25 (header, product table, isydeg records, block
     directory) and ijom byte for byte, the f50 stream of every block and the h_MO record
     within 1e-12 of the flang-built reference's files."""
     torch = pytest.importorskip("torch")
