@@ -143,7 +143,7 @@ int run_half_variant(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* 
                      int store_t) {
     if (precision == CIPSI_PREC_FP64 && !variant_supports(var, row_class(round16(n))))
         return CIPSI_ERR_UNSUPPORTED;
-    if (store_t && (half != 1 || precision != CIPSI_PREC_FP64)) return CIPSI_ERR_ARG;
+    if (store_t && half != 1) return CIPSI_ERR_ARG;
     const int64_t pa = npairs(n), pm = npairs(m);
     HalfArgs a{};
     a.src = src;
@@ -330,7 +330,7 @@ int cipsi_ijkl_synchronize(cipsi_ijkl_ctx* ctx) {
 // lane's stores go to separate 128-byte lines, but each line holds 16 neighbouring AO
 // pairs, which the chunked XCD mapping runs on one L2 at the same time, so the lines are
 // completed there.  That costs half 1 about 2.3 ms at N = 240 and saves the 2.8 ms
-// transpose.  The split path writes H and transposes it.
+// transpose.  The split path's last step of half 1 stores H^T the same way.
 static int transform_front(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* d_ao,
                            const double* d_c, int ldc, int precision, int n_slices,
                            const double** ht, double** cr, bool expanded = false) {
@@ -345,17 +345,8 @@ static int transform_front(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double
         StageTimer tm(ctx, "expand");
         HIP_TRY(ctx, launch_expand_packed(d_ao, pa, vsq, ctx->stream));
     }
-    if (precision == CIPSI_PREC_FP64) {
-        if ((rc = run_half(ctx, 1, n_ao, n_mo, vsq, *cr, h, precision, n_slices, 1))) return rc;
-        *ht = h;
-        return CIPSI_OK;
-    }
-    if ((rc = run_half(ctx, 1, n_ao, n_mo, vsq, *cr, h, precision, n_slices))) return rc;
-    {
-        StageTimer tm(ctx, "transpose");
-        HIP_TRY(ctx, launch_transpose(h, pa, pm, vsq, ctx->stream));   // H^T reuses vsq
-    }
-    *ht = vsq;
+    if ((rc = run_half(ctx, 1, n_ao, n_mo, vsq, *cr, h, precision, n_slices, 1))) return rc;
+    *ht = h;
     return CIPSI_OK;
 }
 

@@ -40,13 +40,14 @@ constexpr int kMaxSlices = 6;
 constexpr int kDefaultSlices = 6;
 constexpr int TR = 32;        // rows of A per work-group
 constexpr int KCH = 32;       // K per chunk (one MFMA)
+constexpr int KST = 2 * KCH;  // K per stage: the chunks staged between two barriers
 constexpr int APAD = 40;      // LDS row stride of a digit plane, in halves
 constexpr int NWAVE = 8;      // 8 waves x 32 columns = up to 256 output columns
 constexpr int NTHR = NWAVE * 64;
 
 typedef __attribute__((address_space(1))) const double gcdouble;
 
-__host__ __device__ inline int round32(int x) { return (x + 31) & ~31; }
+__host__ __device__ inline int round_stage(int x) { return (x + KST - 1) & ~(KST - 1); }
 
 // Exponent e with |x| < 2^e (x != 0).
 __device__ inline int exp_above(double x) { return ilogb(fabs(x)) + 1; }
@@ -65,6 +66,41 @@ __device__ inline void digits(double x, int e, _Float16* d) {
         Y = (Y - v) >> 8;
     }
     d[0] = (_Float16)(float)(int)Y;                               // [-128, 128]
+}
+
+// The same digits for two values at once, as packed pairs of FP16: w[s] holds digit s
+// of x0 in its low half and of x1 in its high half.  Adding 1.5 * 2^52 + B_S, with
+// B_S = 128 (1 + 256 + ... + 256^(S-1)), rounds y to an integer and biases every
+// base-256 field by 128, so that the fields of the low word bits are the balanced digits
+// plus 128 (the top field, 9 bits, is d_0 + 128 in [0, 256]); a field f becomes the FP16
+// value 1024 + f by setting the exponent bits 0x6400, and subtracting 1152 leaves the
+// digit.  Equal to digits<S> element by element.
+template <int S>
+__device__ constexpr double digit_magic() {
+    double b = 0.0, p = 1.0;
+    for (int i = 0; i < S; ++i) {
+        b += 128.0 * p;
+        p *= 256.0;
+    }
+    return 6755399441055744.0 + b;
+}
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+template <int S>
+__device__ inline void digits_pair(double x0, double x1, int e0, int e1, uint32_t (&w)[S]) {
+    constexpr double M = digit_magic<S>();
+    const uint64_t b0 = (uint64_t)__double_as_longlong(ldexp(x0, 8 * S - 1 - e0) + M);
+    const uint64_t b1 = (uint64_t)__double_as_longlong(ldexp(x1, 8 * S - 1 - e1) + M);
+    const half2_t off = half2_t{(_Float16)1152.0f, (_Float16)1152.0f};
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int sh = 8 * (S - 1 - i);                     // digit i sits at bit 8 (S-1-i)
+        const uint32_t mask = i == 0 ? 0x1FFu : 0xFFu;
+        const uint32_t f0 = (uint32_t)(b0 >> sh) & mask, f1 = (uint32_t)(b1 >> sh) & mask;
+        const uint32_t wd = f0 | (f1 << 16) | 0x64006400u;
+        w[i] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(half2_t, wd) - off);
+    }
 }
 
 // C digits: cs[((s * kc + chunk) * mp + c) * 32 + k] for row r = 32 chunk + k, and the
@@ -92,29 +128,38 @@ __global__ void k_split_cslices(const double* __restrict__ cr, int ldm, int np, 
 }
 
 // Row exponents of the symmetric matrices Z_b (rows of the packed source): one
-// work-group per matrix, coalesced over the packed row, LDS atomic max per AO index.
+// work-group per matrix.  Each wave walks whole packed rows r (elements (r, s), s <= r,
+// contiguous), takes the row's maximum |x| by a wave reduction, and hands each element
+// with s < r to row s by an LDS atomic max on distinct addresses.  Maxima are kept as
+// the bits of |x| (ordered like the values), so the exponent is that of the largest
+// element, as element-wise exponents would give.
 __global__ __launch_bounds__(256) void k_split_rowexp(const double* __restrict__ src,
                                                       int64_t src_ld, int n, int kp,
                                                       int* __restrict__ rexp) {
-    __shared__ int emax[256];
+    __shared__ unsigned long long mx[256];
     const int64_t b = blockIdx.x;
-    for (int i = threadIdx.x; i < 256; i += 256) emax[i] = INT_MIN;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    mx[threadIdx.x] = 0ull;
     __syncthreads();
-    const int64_t pa = npairs(n);
-    const double* row = src + b * src_ld;
-    for (int64_t idx = threadIdx.x; idx < pa; idx += 256) {
-        const double x = row[idx];
-        if (x == 0.0) continue;
-        const int r = (int)pair_row(idx);
-        const int s = (int)(idx - pair0(r, 0));
-        const int e = exp_above(x);
-        atomicMax(&emax[r], e);
-        if (s != r) atomicMax(&emax[s], e);
+    const unsigned long long* row = (const unsigned long long*)(src + b * src_ld);
+    for (int r = w; r < n; r += 4) {
+        const unsigned long long* seg = row + pair0(r, 0);
+        unsigned long long m = 0ull;
+        for (int s = lane; s <= r; s += 64) {
+            const unsigned long long v = seg[s] & 0x7FFFFFFFFFFFFFFFull;
+            m = v > m ? v : m;
+            if (s < r) atomicMax(&mx[s], v);
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            const unsigned long long t = __shfl_xor(m, o);
+            m = t > m ? t : m;
+        }
+        if (lane == 0) atomicMax(&mx[r], m);
     }
     __syncthreads();
     for (int r = threadIdx.x; r < kp; r += 256) {
-        const int e = r < n ? emax[r] : INT_MIN;
-        rexp[b * kp + r] = e == INT_MIN ? 0 : e;
+        const unsigned long long v = r < n ? mx[r] : 0ull;
+        rexp[b * kp + r] = v == 0ull ? 0 : exp_above(__longlong_as_double((long long)v));
     }
 }
 
@@ -138,12 +183,13 @@ struct SplitArgs {
     int64_t dst_ld;          // step 2, half 1: P_M
     const int64_t* tile_start;  // step 2, half 2: prefix over l of (l+1) ceil((l+1)/32)
     int64_t nitems;
+    int store_t;             // step 2, half 1: write H^T, element p of row b at p * nb + b
 };
 
 // One work item: 32 rows of A (matrix b) times up to 256 columns of C.
 template <int S, int STEP>
 __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
-    __shared__ _Float16 As[2][S][TR][APAD];
+    __shared__ _Float16 As[2][2][S][TR][APAD];     // [buffer][chunk of the stage][digit]
     __shared__ int rx[TR];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
@@ -157,16 +203,23 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
     int64_t limit = 0, dst_off = 0;
     const int kc = A.kp / KCH;
     if (STEP == 1) {
-        const int ntr = A.kp / TR;
+        const int ntr = (A.n + TR - 1) / TR;
         b = item / ntr;
         rt = (int)(item % ntr);
         nrows = A.n;
         ncols = A.half == 1 ? A.m : (int)pair_row(b) + 1;        // half 2: columns j <= l
     } else {
         if (A.half == 1) {
+            // with H^T stores, consecutive items take consecutive matrices b of one row
+            // tile: the XCD chunk of an item then fills whole 128-byte lines of H^T
             const int ntr = (A.m + TR - 1) / TR;
-            b = item / ntr;
-            rt = (int)(item % ntr);
+            if (A.store_t) {
+                rt = (int)(item / A.nb);
+                b = item - (int64_t)rt * A.nb;
+            } else {
+                b = item / ntr;
+                rt = (int)(item % ntr);
+            }
             nrows = A.m;
             limit = A.dst_ld;
             dst_off = b * A.dst_ld;
@@ -203,17 +256,15 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
     }
     __syncthreads();
 
-    // ---- A chunk staging: element u of this thread -> (row i, k)
-    constexpr int EPT = TR * KCH / NTHR;                            // 2
-    double xs[EPT];
+    // ---- A chunk staging: this thread's two elements (row si, k = sk and sk + 1)
+    static_assert(TR * KCH == 2 * NTHR, "two elements per thread and chunk");
+    const int si = tid / (KCH / 2), sk = 2 * (tid % (KCH / 2));
     gcdouble* __restrict__ src = (gcdouble*)A.src;
     // Loads are unconditional (an element outside the matrix reads element 0 of its row,
     // which exists) and masked at commit, so no register is written by anything but its
     // load and the loop body has no branches around loads.
     auto elem = [&](int u, int ch, int64_t& idx) -> bool {
-        const int e = tid + u * NTHR;
-        const int i = e / KCH, k = e % KCH;
-        const int r = row0 + i, s = ch * KCH + k;
+        const int r = row0 + si, s = ch * KCH + sk + u;
         bool ok;
         if (STEP == 1) {
             ok = r < A.n && s < A.n;
@@ -225,25 +276,25 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
         }
         return ok;
     };
-    auto fetch = [&](int ch) {
+    auto fetch = [&](int st, double (&xs)[4]) {
 #pragma unroll
-        for (int u = 0; u < EPT; ++u) {
-            int64_t idx;
-            (void)elem(u, ch, idx);
-            xs[u] = src[idx];
-        }
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                int64_t idx;
+                (void)elem(u, 2 * st + c, idx);
+                xs[2 * c + u] = src[idx];
+            }
     };
-    auto commit = [&](int ch, int buf) {
+    auto commit = [&](int st, int buf, const double (&xs)[4]) {
 #pragma unroll
-        for (int u = 0; u < EPT; ++u) {
-            const int e = tid + u * NTHR;
-            const int i = e / KCH, k = e % KCH;
+        for (int c = 0; c < 2; ++c) {
             int64_t idx;
-            const bool ok = elem(u, ch, idx);
-            _Float16 d[S];
-            digits<S>(ok ? xs[u] : 0.0, rx[i], d);
+            const bool ok0 = elem(0, 2 * st + c, idx), ok1 = elem(1, 2 * st + c, idx);
+            uint32_t w[S];
+            digits_pair<S>(ok0 ? xs[2 * c] : 0.0, ok1 ? xs[2 * c + 1] : 0.0, rx[si], rx[si], w);
 #pragma unroll
-            for (int s = 0; s < S; ++s) As[buf][s][i][k] = d[s];
+            for (int s = 0; s < S; ++s) *(uint32_t*)&As[buf][c][s][si][sk] = w[s];
         }
     };
 
@@ -255,9 +306,6 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
 #pragma unroll
             for (int j = 0; j < 2; ++j) acc[g][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    typedef __attribute__((address_space(1))) const f16x8 gcf16x8;
-    gcf16x8* __restrict__ csv = (gcf16x8*)A.cs;
-
     // B fragments (digits of C), double-buffered in registers: the next chunk's S x 2
     // fragments are loaded at the start of a chunk, ahead of its A prefetch, by every
     // wave (a wave without columns reads a clamped column), so that no load sits in a
@@ -265,24 +313,32 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
     // chunk's MFMAs wait only for B loads issued a whole chunk earlier.  Columns past mp
     // only feed outputs that are never stored.  The A fragments are read from LDS per
     // digit i, so only one pair is live.
+    // The loads go through a buffer resource: the lane's column offset is one VGPR per
+    // column block, and the digit plane and chunk a scalar offset.
     f16x8 bcur[S][2], bnext[S][2];
+    const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)A.cs, (short)0, (int)(sizeof(_Float16) * (size_t)S * A.kp * A.mp), 0x00020000);
+    unsigned boff[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+        boff[cb] = (unsigned)(min(cw + 16 * cb + lm, A.mp - 1) * KCH + 8 * lg) * 2u;
     auto load_b = [&](int ch, f16x8 (&bf)[S][2]) {
 #pragma unroll
-        for (int j = 0; j < S; ++j)
+        for (int j = 0; j < S; ++j) {
+            const unsigned so = (unsigned)(((j * kc + ch) * A.mp) * KCH) * 2u;
 #pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
-                const int c = min(cw + 16 * cb + lm, A.mp - 1);
-                bf[j][cb] = csv[(((int64_t)j * kc + ch) * A.mp + c) * (KCH / 8) + lg];
-            }
+            for (int cb = 0; cb < 2; ++cb)
+                bf[j][cb] = __builtin_bit_cast(
+                    f16x8, __builtin_amdgcn_raw_buffer_load_b128(crs, boff[cb], so, 0));
+        }
     };
-    auto mfma_chunk = [&](int ch) {
-        const int buf = ch & 1;
+    auto mfma_chunk = [&](int buf, int c, const f16x8 (&bc)[S][2]) {
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             f16x8 af[2];
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb)
-                af[rb] = *(const f16x8*)&As[buf][i][16 * rb + lm][8 * lg];
+                af[rb] = *(const f16x8*)&As[buf][c][i][16 * rb + lm][8 * lg];
 #pragma unroll
             for (int j = 0; i + j < S; ++j)
 #pragma unroll
@@ -290,30 +346,38 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
 #pragma unroll
                     for (int cb = 0; cb < 2; ++cb)
                         acc[i + j][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                            af[rb], bcur[j][cb], acc[i + j][rb][cb], 0, 0, 0);
+                            af[rb], bc[j][cb], acc[i + j][rb][cb], 0, 0, 0);
         }
     };
+    // K runs in stages of two chunks, one barrier per stage.  The A elements are fetched
+    // two stages ahead (their rows mostly come from HBM), the B fragments one chunk ahead
+    // (they stay in L2).  Fetches past the last stage or chunk are clamped to it, so that
+    // no load is conditional.
+    const int nst = kc / 2;                        // kp is a whole number of stages
+    double xa[4], xb[4];
     load_b(0, bcur);
-    fetch(0);
-    commit(0, 0);
-    __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0): nothing pending into the loop
+    fetch(0, xa);
+    commit(0, 0, xa);
+    fetch(nst > 1 ? 1 : 0, xb);
     __syncthreads();
-    // all chunks but the last prefetch the next chunk's B and A; the last chunk is peeled
-    // so that the loop body has no conditional loads
-    for (int ch = 0; ch + 1 < kc; ++ch) {
-        load_b(ch + 1, bnext);
-        fetch(ch + 1);
+    // the two B buffers alternate between the two chunks of a stage: bcur feeds the even
+    // chunk and is reloaded for the next stage as soon as its MFMAs are issued
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        fetch(min(st + 2, nst - 1), xa);
+        load_b(2 * st + 1, bnext);
         __builtin_amdgcn_sched_barrier(0);
-        if (active) mfma_chunk(ch);
+        if (active) mfma_chunk(buf, 0, bcur);
         __builtin_amdgcn_sched_barrier(0);
-        commit(ch + 1, (ch + 1) & 1);
+        load_b(min(2 * st + 2, kc - 1), bcur);
+        __builtin_amdgcn_sched_barrier(0);
+        if (active) mfma_chunk(buf, 1, bnext);
+        __builtin_amdgcn_sched_barrier(0);
+        if (st + 1 < nst) commit(st + 1, buf ^ 1, xb);
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < S; ++j)
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb) bcur[j][cb] = bnext[j][cb];
+        for (int u = 0; u < 4; ++u) xb[u] = xa[u];
     }
-    if (active) mfma_chunk(kc - 1);
     if (!active) return;
 
     // ---- epilogue: FP64 recombination and stores
@@ -343,7 +407,10 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
                 } else {
                     if (r < nrows && col <= r) {
                         const int64_t p = pair0(r, col);
-                        if (p < limit) dst[dst_off + p] = v;
+                        if (p < limit) {
+                            if (A.store_t) dst[p * A.nb + b] = v;
+                            else dst[dst_off + p] = v;
+                        }
                     }
                 }
             }
@@ -398,7 +465,7 @@ struct SplitLayout {
 SplitLayout split_layout(int n, int m, int s) {
     const int64_t pa = npairs(n), pm = npairs(m);
     const int64_t nb = pa > pm ? pa : pm;
-    const int kp = round32(n), mp = (m + 15) & ~15;
+    const int kp = round_stage(n), mp = (m + 15) & ~15;
     SplitLayout L;
     size_t off = 0;
     L.u = off;
@@ -435,7 +502,7 @@ double split_unit_roundoff(int, int n_slices) {
 int run_half_split(cipsi_ijkl_ctx* ctx, const HalfArgs& a, int, int n_slices) {
     const int s = n_slices == 0 ? kDefaultSlices : n_slices;
     if (s < kMinSlices || s > kMaxSlices) return CIPSI_ERR_UNSUPPORTED;
-    const int n = a.n, m = a.m, mp = a.mp, kp = round32(n);
+    const int n = a.n, m = a.m, mp = a.mp, kp = round_stage(n);
     if (mp > NWAVE * 32 || kp > 256) return CIPSI_ERR_UNSUPPORTED;
     const SplitLayout L = split_layout(n, m, kMaxSlices);
     int rc = ctx_alloc(ctx, ctx->slices, L.total);
@@ -470,7 +537,7 @@ int run_half_split(cipsi_ijkl_ctx* ctx, const HalfArgs& a, int, int n_slices) {
     x.rexp = rexp;
     x.dst = u;
     x.dexp = uexp;
-    x.nitems = a.nb * (int64_t)(kp / TR);
+    x.nitems = a.nb * (int64_t)((n + TR - 1) / TR);
     if ((e = launch_split_s<1>(s, x, st)) != hipSuccess) return CIPSI_ERR_HIP;
     // step 2: O_b^T = U_b C, packed triangle
     x.src = u;
@@ -480,6 +547,7 @@ int run_half_split(cipsi_ijkl_ctx* ctx, const HalfArgs& a, int, int n_slices) {
     x.dexp = nullptr;
     if (a.half == 1) {
         x.dst_ld = a.dst_ld;
+        x.store_t = a.store_t;
         x.tile_start = nullptr;
         x.nitems = a.nb * (int64_t)((m + TR - 1) / TR);
     } else {
