@@ -82,16 +82,17 @@ static int check_dims(int n_ao, int n_mo, int ldc) {
 
 static inline int round16(int x) { return (x + 15) & ~15; }
 
-// Half-2 work table: tile_start[l] = sum_{l'<l} (l'+1) * ceil((l'+1)/lt).  One table per
-// (m, lt), built on first use (a synchronous upload, so not under graph capture) and never
-// rewritten, so a captured graph keeps a valid table whatever later calls use.
-int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int64_t* total) {
-    const auto key = std::make_pair(m, lt);
+// Half-2 work tables: tile_start[l] = sum_{l'<l} (l'+1) * items(l'), items(l) the work
+// items of one MO pair of row l.  One table per key, built on first use (a synchronous
+// upload, so not under graph capture) and never rewritten, so a captured graph keeps a
+// valid table whatever later calls use.
+int ctx_prefix_table(cipsi_ijkl_ctx* ctx, std::pair<int, int> key, int m, int64_t (*items)(int, int),
+                     const int64_t** out, int64_t* total) {
     auto it = ctx->tile_tables.find(key);
     if (it == ctx->tile_tables.end()) {
         std::vector<int64_t> t(m + 1);
         t[0] = 0;
-        for (int l = 0; l < m; ++l) t[l + 1] = t[l] + (int64_t)(l + 1) * ((l + lt) / lt);
+        for (int l = 0; l < m; ++l) t[l + 1] = t[l] + (int64_t)(l + 1) * items(l, key.second);
         cipsi_ijkl_ctx::TileTable tab;
         int rc = ctx_alloc(ctx, tab.buf, sizeof(int64_t) * (m + 1));
         if (rc) return rc;
@@ -112,6 +113,13 @@ int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int6
     *out = (const int64_t*)it->second.buf.ptr;
     *total = it->second.total;
     return CIPSI_OK;
+}
+
+// FP64 half 2: an MO pair of row l has ceil((l+1)/lt) column tiles of lt columns.
+static int64_t column_tiles(int l, int lt) { return (l + lt) / lt; }
+
+int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int64_t* total) {
+    return ctx_prefix_table(ctx, std::make_pair(m, lt), m, column_tiles, out, total);
 }
 
 int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half, int n, int m, int precision) {

@@ -102,6 +102,10 @@ int ctx_alloc(cipsi_ijkl_ctx* ctx, DevBuf& b, size_t bytes);
 int ctx_copy_h2d(cipsi_ijkl_ctx* ctx, void* d, const void* h, size_t bytes);
 int ctx_enter(cipsi_ijkl_ctx* ctx);
 int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int64_t* total);
+// A half-2 prefix table keyed by (m, kind): kind > 0 is the FP64 column width; the split
+// path uses kind < 0.  items(l, kind) is the number of work items of one MO pair of row l.
+int ctx_prefix_table(cipsi_ijkl_ctx* ctx, std::pair<int, int> key, int m, int64_t (*items)(int, int),
+                     const int64_t** out, int64_t* total);
 int run_pad_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_c, int ldc, double** cr);
 // store_t (FP64 half 1): write H^T, rows of P_A per MO pair, instead of H.
 int run_half(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* src, const double* cr,

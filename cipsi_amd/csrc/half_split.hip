@@ -38,16 +38,41 @@ namespace {
 constexpr int kMinSlices = 3;
 constexpr int kMaxSlices = 6;
 constexpr int kDefaultSlices = 6;
-constexpr int TR = 32;        // rows of A per work-group
+constexpr int TR = 32;        // rows of A per work item
 constexpr int KCH = 32;       // K per chunk (one MFMA)
 constexpr int KST = 2 * KCH;  // K per stage: the chunks staged between two barriers
 constexpr int APAD = 40;      // LDS row stride of a digit plane, in halves
-constexpr int NWAVE = 8;      // 8 waves x 32 columns = up to 256 output columns
-constexpr int NTHR = NWAVE * 64;
+constexpr int MAXC = 256;     // output columns of a matrix (N_MO rounded up to 16)
+// Waves per work-group: each wave owns 32 output columns.  Step 1 runs one 8-wave
+// work-group per CU over all columns of a row tile (the A digits in LDS serve 8 waves);
+// step 2, whose triangle leaves many columns empty, runs two 4-wave work-groups per CU
+// over column tiles of 128, so idle column tiles cost nothing and one work-group's
+// barriers and epilogue overlap the other's MFMAs.
+constexpr int NW1 = 8;
+constexpr int NW2 = 4;
+constexpr int TC = NW2 * 32;  // step-2 column tile
 
 typedef __attribute__((address_space(1))) const double gcdouble;
 
 __host__ __device__ inline int round_stage(int x) { return (x + KST - 1) & ~(KST - 1); }
+
+// Step 2 keeps a <= c only: row tile rt (c < TR (rt + 1)) has tri_cols(rt) column tiles,
+// and a matrix of nrt row tiles has tri_items(nrt) work items, numbered row tile by row
+// tile.
+__host__ __device__ inline int tri_cols(int rt) { return (TR * (rt + 1) + TC - 1) / TC; }
+__host__ __device__ inline int tri_items(int nrt) {
+    int t = 0;
+    for (int r = 0; r < nrt; ++r) t += tri_cols(r);
+    return t;
+}
+__device__ inline void tri_item(int p, int& rt, int& ct) {
+    rt = 0;
+    while (p >= tri_cols(rt)) {
+        p -= tri_cols(rt);
+        ++rt;
+    }
+    ct = p;
+}
 
 // Exponent e with |x| < 2^e (x != 0).
 __device__ inline int exp_above(double x) { return ilogb(fabs(x)) + 1; }
@@ -186,9 +211,18 @@ struct SplitArgs {
     int store_t;             // step 2, half 1: write H^T, element p of row b at p * nb + b
 };
 
-// One work item: 32 rows of A (matrix b) times up to 256 columns of C.
-template <int S, int STEP>
-__global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
+// One work item: TR rows of A (matrix b) times the NW x 32 columns of column tile ct.
+//   step 1  item = ((b * ntr + rt) * MAXC / (NW x 32) + ct); a column tile past the
+//           matrix's columns has no work and returns at once;
+//   step 2  the (rt, ct) pairs of the triangle a <= c, tri_items per matrix: half 1 takes
+//           consecutive matrices b for one pair (H^T stores: an XCD's chunk of items fills
+//           whole 128-byte lines) or consecutive pairs of one b; half 2 finds the MO pair
+//           row l in the prefix table.
+template <int S, int STEP, int NW>
+__global__ __launch_bounds__(NW * 64, 8 / NW) void k_split_gemm(SplitArgs A) {
+    constexpr int NTHR = NW * 64, TCW = NW * 32, NCT = MAXC / TCW;
+    constexpr int EPT = TR * KCH / NTHR;           // A elements per thread and chunk
+    static_assert(EPT == 2 || EPT == 4, "two or four A elements per thread and chunk");
     __shared__ _Float16 As[2][2][S][TR][APAD];     // [buffer][chunk of the stage][digit]
     __shared__ int rx[TR];
     const int tid = threadIdx.x;
@@ -197,28 +231,29 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
     const int64_t item = xcd_item(blockIdx.x);
     if (item >= A.nitems) return;
 
-    // ---- work item -> matrix b, row tile, extents
+    // ---- work item -> matrix b, row tile, column tile, extents
     int64_t b;
-    int rt, nrows, ncols;
+    int rt, ct, nrows, ncols;
     int64_t limit = 0, dst_off = 0;
     const int kc = A.kp / KCH;
     if (STEP == 1) {
         const int ntr = (A.n + TR - 1) / TR;
-        b = item / ntr;
-        rt = (int)(item % ntr);
+        const int64_t bt = item / NCT;
+        ct = (int)(item - bt * NCT);
+        b = bt / ntr;
+        rt = (int)(bt - b * ntr);
         nrows = A.n;
         ncols = A.half == 1 ? A.m : (int)pair_row(b) + 1;        // half 2: columns j <= l
     } else {
+        int p;
         if (A.half == 1) {
-            // with H^T stores, consecutive items take consecutive matrices b of one row
-            // tile: the XCD chunk of an item then fills whole 128-byte lines of H^T
-            const int ntr = (A.m + TR - 1) / TR;
+            const int npm = tri_items((A.m + TR - 1) / TR);
             if (A.store_t) {
-                rt = (int)(item / A.nb);
-                b = item - (int64_t)rt * A.nb;
+                p = (int)(item / A.nb);
+                b = item - (int64_t)p * A.nb;
             } else {
-                b = item / ntr;
-                rt = (int)(item % ntr);
+                b = item / npm;
+                p = (int)(item - b * npm);
             }
             nrows = A.m;
             limit = A.dst_ld;
@@ -230,35 +265,24 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
                 if (A.tile_start[mid] <= item) lo = mid; else hi = mid;
             }
             const int l = lo;
-            const int nt = (l + TR) / TR;
+            const int npm = tri_items((l + TR) / TR);
             const int64_t rem = item - A.tile_start[l];
-            b = pair0(l, 0) + rem / nt;
-            rt = (int)(rem % nt);
+            b = pair0(l, 0) + rem / npm;
+            p = (int)(rem % npm);
             nrows = l + 1;
             limit = b + 1;
             dst_off = pair0(b, 0);
         }
+        tri_item(p, rt, ct);
         ncols = min(rt * TR + TR, nrows);                           // a <= c
     }
+    if (ct * TCW >= ncols) return;                                  // uniform: no columns
     const int row0 = rt * TR;
-    const int cw = 32 * w;                                          // this wave's columns
+    const int cw = ct * TCW + 32 * w;                               // this wave's columns
     const bool active = cw < ncols;
 
-    if (tid < TR) {
-        const int r = row0 + tid;
-        int e = 0;
-        if (STEP == 1) e = r < A.n ? A.rexp[b * A.kp + r] : 0;
-        else if (r < nrows) {
-            e = A.rexp[b * A.mp + r];
-            if (e == INT_MIN) e = 0;
-        }
-        rx[tid] = e;
-    }
-    __syncthreads();
-
-    // ---- A chunk staging: this thread's two elements (row si, k = sk and sk + 1)
-    static_assert(TR * KCH == 2 * NTHR, "two elements per thread and chunk");
-    const int si = tid / (KCH / 2), sk = 2 * (tid % (KCH / 2));
+    // ---- A chunk staging: this thread's EPT elements (row si, k = sk .. sk + EPT - 1)
+    const int si = tid / (KCH / EPT), sk = EPT * (tid % (KCH / EPT));
     gcdouble* __restrict__ src = (gcdouble*)A.src;
     // Loads are unconditional (an element outside the matrix reads element 0 of its row,
     // which exists) and masked at commit, so no register is written by anything but its
@@ -276,25 +300,36 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
         }
         return ok;
     };
-    auto fetch = [&](int st, double (&xs)[4]) {
+    auto fetch = [&](int st, double (&xs)[2 * EPT]) {
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < EPT; ++u) {
                 int64_t idx;
                 (void)elem(u, 2 * st + c, idx);
-                xs[2 * c + u] = src[idx];
+                xs[EPT * c + u] = src[idx];
             }
     };
-    auto commit = [&](int st, int buf, const double (&xs)[4]) {
+    auto commit = [&](int st, int buf, const double (&xs)[2 * EPT]) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            int64_t idx;
-            const bool ok0 = elem(0, 2 * st + c, idx), ok1 = elem(1, 2 * st + c, idx);
-            uint32_t w[S];
-            digits_pair<S>(ok0 ? xs[2 * c] : 0.0, ok1 ? xs[2 * c + 1] : 0.0, rx[si], rx[si], w);
+            double v[EPT];
 #pragma unroll
-            for (int s = 0; s < S; ++s) *(uint32_t*)&As[buf][c][s][si][sk] = w[s];
+            for (int u = 0; u < EPT; ++u) {
+                int64_t idx;
+                v[u] = elem(u, 2 * st + c, idx) ? xs[EPT * c + u] : 0.0;
+            }
+            uint32_t w0[S], w1[S];
+            digits_pair<S>(v[0], v[1], rx[si], rx[si], w0);
+            if constexpr (EPT == 4) {
+                digits_pair<S>(v[2], v[3], rx[si], rx[si], w1);
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    *(uint2*)&As[buf][c][s][si][sk] = make_uint2(w0[s], w1[s]);
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) *(uint32_t*)&As[buf][c][s][si][sk] = w0[s];
+            }
         }
     };
 
@@ -354,29 +389,45 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
     // (they stay in L2).  Fetches past the last stage or chunk are clamped to it, so that
     // no load is conditional.
     const int nst = kc / 2;                        // kp is a whole number of stages
-    double xa[4], xb[4];
+    double xa[2 * EPT], xb[2 * EPT];
+    // The first loads go out before the row exponents are read, so that their latencies
+    // overlap.
     load_b(0, bcur);
     fetch(0, xa);
-    commit(0, 0, xa);
     fetch(nst > 1 ? 1 : 0, xb);
+    if (tid < TR) {
+        const int r = row0 + tid;
+        int e = 0;
+        if (STEP == 1) e = r < A.n ? A.rexp[b * A.kp + r] : 0;
+        else if (r < nrows) {
+            e = A.rexp[b * A.mp + r];
+            if (e == INT_MIN) e = 0;
+        }
+        rx[tid] = e;
+    }
     __syncthreads();
-    // the two B buffers alternate between the two chunks of a stage: bcur feeds the even
-    // chunk and is reloaded for the next stage as soon as its MFMAs are issued
+    commit(0, 0, xa);
+    __syncthreads();
+    // The two B buffers alternate between the two chunks of a stage: bcur feeds the even
+    // chunk and is reloaded for the next stage as soon as its MFMAs are issued.  Loads
+    // complete in order, so the A prefetch is issued after that reload: the odd chunk's
+    // MFMAs wait for B loads issued before it and not for the HBM gathers, which have
+    // until the next stage's odd chunk.
     for (int st = 0; st < nst; ++st) {
         const int buf = st & 1;
-        fetch(min(st + 2, nst - 1), xa);
         load_b(2 * st + 1, bnext);
         __builtin_amdgcn_sched_barrier(0);
         if (active) mfma_chunk(buf, 0, bcur);
         __builtin_amdgcn_sched_barrier(0);
         load_b(min(2 * st + 2, kc - 1), bcur);
+        fetch(min(st + 2, nst - 1), xa);
         __builtin_amdgcn_sched_barrier(0);
         if (active) mfma_chunk(buf, 1, bnext);
         __builtin_amdgcn_sched_barrier(0);
         if (st + 1 < nst) commit(st + 1, buf ^ 1, xb);
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xb[u] = xa[u];
+        for (int u = 0; u < 2 * EPT; ++u) xb[u] = xa[u];
     }
     if (!active) return;
 
@@ -426,9 +477,10 @@ __global__ __launch_bounds__(NTHR) void k_split_gemm(SplitArgs A) {
 
 template <int S, int STEP>
 hipError_t launch_split_gemm(const SplitArgs& a, hipStream_t st) {
+    constexpr int NW = STEP == 1 ? NW1 : NW2;
     const int64_t g = xcd_grid(a.nitems);
     if (g <= 0 || g > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_split_gemm<S, STEP>), dim3((unsigned)g), dim3(NTHR), 0, st, a);
+    hipLaunchKernelGGL((k_split_gemm<S, STEP, NW>), dim3((unsigned)g), dim3(NW * 64), 0, st, a);
     return hipGetLastError();
 }
 
@@ -499,11 +551,14 @@ double split_unit_roundoff(int, int n_slices) {
     return std::ldexp(1.0, 1 - 8 * s);
 }
 
+// Step-2 work items of one MO pair of row l in half 2 (its rows c <= l).
+static int64_t step2_items(int l, int) { return tri_items((l + TR) / TR); }
+
 int run_half_split(cipsi_ijkl_ctx* ctx, const HalfArgs& a, int, int n_slices) {
     const int s = n_slices == 0 ? kDefaultSlices : n_slices;
     if (s < kMinSlices || s > kMaxSlices) return CIPSI_ERR_UNSUPPORTED;
     const int n = a.n, m = a.m, mp = a.mp, kp = round_stage(n);
-    if (mp > NWAVE * 32 || kp > 256) return CIPSI_ERR_UNSUPPORTED;
+    if (mp > MAXC || kp > 256) return CIPSI_ERR_UNSUPPORTED;
     const SplitLayout L = split_layout(n, m, kMaxSlices);
     int rc = ctx_alloc(ctx, ctx->slices, L.total);
     if (rc) return rc;
@@ -537,7 +592,7 @@ int run_half_split(cipsi_ijkl_ctx* ctx, const HalfArgs& a, int, int n_slices) {
     x.rexp = rexp;
     x.dst = u;
     x.dexp = uexp;
-    x.nitems = a.nb * (int64_t)((n + TR - 1) / TR);
+    x.nitems = a.nb * (int64_t)((n + TR - 1) / TR) * (MAXC / (NW1 * 32));
     if ((e = launch_split_s<1>(s, x, st)) != hipSuccess) return CIPSI_ERR_HIP;
     // step 2: O_b^T = U_b C, packed triangle
     x.src = u;
@@ -549,11 +604,11 @@ int run_half_split(cipsi_ijkl_ctx* ctx, const HalfArgs& a, int, int n_slices) {
         x.dst_ld = a.dst_ld;
         x.store_t = a.store_t;
         x.tile_start = nullptr;
-        x.nitems = a.nb * (int64_t)((m + TR - 1) / TR);
+        x.nitems = a.nb * (int64_t)tri_items((m + TR - 1) / TR);
     } else {
         const int64_t* ts;
         int64_t total;
-        if ((rc = ctx_tile_table(ctx, m, TR, &ts, &total))) return rc;
+        if ((rc = ctx_prefix_table(ctx, std::make_pair(m, -1), m, step2_items, &ts, &total))) return rc;
         x.tile_start = ts;
         x.nitems = total;
     }
