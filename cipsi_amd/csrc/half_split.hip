@@ -1,10 +1,13 @@
 // Split-FP16 (Ozaki) half-transform on FP16 MFMA with FP32 accumulation.
 //
 // Each half (tpqkld.f:82-97 / tijkld.f:53-76) is run as two slice products:
-//   step 1 (Q1/Q3)  T_b = Z_b C       Z_b rows gathered from the packed symmetric rows;
-//                                     stored transposed, U[b][c][r] = T_b(r, c)
-//   step 2 (Q2/Q4)  O_b^T = U_b C     rows (b, c) of U; only a <= c is kept and written
-//                                     to the packed output exactly as the FP64 kernel does.
+//   step 1 (Q1/Q3)  T_b = Z_b C       k_split_rows, one work-group per matrix b: Z_b rows
+//                                     gathered from the packed symmetric rows; stored
+//                                     transposed, U[b][c][r] = T_b(r, c), with the row
+//                                     exponents of U
+//   step 2 (Q2/Q4)  O_b^T = U_b C     k_split_tri: rows (b, c) of U; only a <= c is kept
+//                                     and written to the packed output exactly as the
+//                                     FP64 kernel does.
 // In both the left operand is a tall FP64 matrix A with K = N_AO <= 256 columns and the
 // right operand is C.  Every row of A (and every column of C) is scaled by a power of
 // two 2^e with |row| < 2^e and rounded to an integer Y of 8S-1 bits,
@@ -188,12 +191,6 @@ __global__ __launch_bounds__(256) void k_split_rowexp(const double* __restrict__
     }
 }
 
-__global__ void k_fill_int(int* __restrict__ p, int64_t n, int v) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = v;
-}
-
 struct SplitArgs {
     const double* src;       // step 1: packed symmetric rows; step 2: U rows (length kp)
     int64_t src_ld;
@@ -201,28 +198,26 @@ struct SplitArgs {
     const _Float16* cs;      // C digits
     const int* cexp;         // C column exponents
     double* dst;             // step 1: U [b][mp][kp]; step 2: packed output
-    int* dexp;               // step 1: U row exponents [b][mp] (atomic max)
+    int* dexp;               // step 1: U row exponents [b][mp]
     int n, m, mp, kp;
     int half;
     int64_t nb;
     int64_t dst_ld;          // step 2, half 1: P_M
-    const int64_t* tile_start;  // step 2, half 2: prefix over l of (l+1) ceil((l+1)/32)
+    const int64_t* tile_start;  // step 2, half 2: prefix over l of (l+1) step2_items(l)
     int64_t nitems;
     int store_t;             // step 2, half 1: write H^T, element p of row b at p * nb + b
 };
 
-// One work item: TR rows of A (matrix b) times the NW x 32 columns of column tile ct.
-//   step 1  item = ((b * ntr + rt) * MAXC / (NW x 32) + ct); a column tile past the
-//           matrix's columns has no work and returns at once;
-//   step 2  the (rt, ct) pairs of the triangle a <= c, tri_items per matrix: half 1 takes
-//           consecutive matrices b for one pair (H^T stores: an XCD's chunk of items fills
-//           whole 128-byte lines) or consecutive pairs of one b; half 2 finds the MO pair
-//           row l in the prefix table.
-template <int S, int STEP, int NW>
-__global__ __launch_bounds__(NW * 64, 8 / NW) void k_split_gemm(SplitArgs A) {
-    constexpr int NTHR = NW * 64, TCW = NW * 32, NCT = MAXC / TCW;
+// Step 2: one work item is TR rows c of U_b times the TC columns a of column tile ct,
+// for the (rt, ct) pairs of the triangle a <= c (tri_items per matrix).  Half 1 takes
+// consecutive matrices b for one pair (H^T stores: an XCD's chunk of items fills whole
+// 128-byte lines) or consecutive pairs of one b; half 2 finds the MO pair row l in the
+// prefix table.  Two 4-wave work-groups run per CU.
+template <int S>
+__global__ __launch_bounds__(NW2 * 64, 2) void k_split_tri(SplitArgs A) {
+    constexpr int NTHR = NW2 * 64;
     constexpr int EPT = TR * KCH / NTHR;           // A elements per thread and chunk
-    static_assert(EPT == 2 || EPT == 4, "two or four A elements per thread and chunk");
+    static_assert(EPT == 4, "four A elements per thread and chunk");
     __shared__ _Float16 As[2][2][S][TR][APAD];     // [buffer][chunk of the stage][digit]
     __shared__ int rx[TR];
     const int tid = threadIdx.x;
@@ -232,53 +227,40 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void k_split_gemm(SplitArgs A) {
     if (item >= A.nitems) return;
 
     // ---- work item -> matrix b, row tile, column tile, extents
-    int64_t b;
-    int rt, ct, nrows, ncols;
-    int64_t limit = 0, dst_off = 0;
+    int64_t b, limit, dst_off;
+    int rt, ct, nrows, p;
     const int kc = A.kp / KCH;
-    if (STEP == 1) {
-        const int ntr = (A.n + TR - 1) / TR;
-        const int64_t bt = item / NCT;
-        ct = (int)(item - bt * NCT);
-        b = bt / ntr;
-        rt = (int)(bt - b * ntr);
-        nrows = A.n;
-        ncols = A.half == 1 ? A.m : (int)pair_row(b) + 1;        // half 2: columns j <= l
-    } else {
-        int p;
-        if (A.half == 1) {
-            const int npm = tri_items((A.m + TR - 1) / TR);
-            if (A.store_t) {
-                p = (int)(item / A.nb);
-                b = item - (int64_t)p * A.nb;
-            } else {
-                b = item / npm;
-                p = (int)(item - b * npm);
-            }
-            nrows = A.m;
-            limit = A.dst_ld;
-            dst_off = b * A.dst_ld;
+    if (A.half == 1) {
+        const int npm = tri_items((A.m + TR - 1) / TR);
+        if (A.store_t) {
+            p = (int)(item / A.nb);
+            b = item - (int64_t)p * A.nb;
         } else {
-            int lo = 0, hi = A.m;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (A.tile_start[mid] <= item) lo = mid; else hi = mid;
-            }
-            const int l = lo;
-            const int npm = tri_items((l + TR) / TR);
-            const int64_t rem = item - A.tile_start[l];
-            b = pair0(l, 0) + rem / npm;
-            p = (int)(rem % npm);
-            nrows = l + 1;
-            limit = b + 1;
-            dst_off = pair0(b, 0);
+            b = item / npm;
+            p = (int)(item - b * npm);
         }
-        tri_item(p, rt, ct);
-        ncols = min(rt * TR + TR, nrows);                           // a <= c
+        nrows = A.m;
+        limit = A.dst_ld;
+        dst_off = b * A.dst_ld;
+    } else {
+        int lo = 0, hi = A.m;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (A.tile_start[mid] <= item) lo = mid; else hi = mid;
+        }
+        const int l = lo;
+        const int npm = tri_items((l + TR) / TR);
+        const int64_t rem = item - A.tile_start[l];
+        b = pair0(l, 0) + rem / npm;
+        p = (int)(rem % npm);
+        nrows = l + 1;
+        limit = b + 1;
+        dst_off = pair0(b, 0);
     }
-    if (ct * TCW >= ncols) return;                                  // uniform: no columns
+    tri_item(p, rt, ct);
+    const int ncols = min(rt * TR + TR, nrows);                     // a <= c
     const int row0 = rt * TR;
-    const int cw = ct * TCW + 32 * w;                               // this wave's columns
+    const int cw = ct * TC + 32 * w;                                // this wave's columns
     const bool active = cw < ncols;
 
     // ---- A chunk staging: this thread's EPT elements (row si, k = sk .. sk + EPT - 1)
@@ -289,15 +271,8 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void k_split_gemm(SplitArgs A) {
     // load and the loop body has no branches around loads.
     auto elem = [&](int u, int ch, int64_t& idx) -> bool {
         const int r = row0 + si, s = ch * KCH + sk + u;
-        bool ok;
-        if (STEP == 1) {
-            ok = r < A.n && s < A.n;
-            const int hi = r >= s ? r : s, lo = r >= s ? s : r;
-            idx = b * A.src_ld + (ok ? pair0(hi, lo) : 0);
-        } else {
-            ok = r < nrows && s < A.n;
-            idx = ok ? (b * A.mp + r) * (int64_t)A.kp + s : b * A.mp * (int64_t)A.kp;
-        }
+        const bool ok = r < nrows && s < A.n;
+        idx = ok ? (b * A.mp + r) * (int64_t)A.kp + s : b * A.mp * (int64_t)A.kp;
         return ok;
     };
     auto fetch = [&](int st, double (&xs)[2 * EPT]) {
@@ -321,15 +296,9 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void k_split_gemm(SplitArgs A) {
             }
             uint32_t w0[S], w1[S];
             digits_pair<S>(v[0], v[1], rx[si], rx[si], w0);
-            if constexpr (EPT == 4) {
-                digits_pair<S>(v[2], v[3], rx[si], rx[si], w1);
+            digits_pair<S>(v[2], v[3], rx[si], rx[si], w1);
 #pragma unroll
-                for (int s = 0; s < S; ++s)
-                    *(uint2*)&As[buf][c][s][si][sk] = make_uint2(w0[s], w1[s]);
-            } else {
-#pragma unroll
-                for (int s = 0; s < S; ++s) *(uint32_t*)&As[buf][c][s][si][sk] = w0[s];
-            }
+            for (int s = 0; s < S; ++s) *(uint2*)&As[buf][c][s][si][sk] = make_uint2(w0[s], w1[s]);
         }
     };
 
@@ -398,8 +367,7 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void k_split_gemm(SplitArgs A) {
     if (tid < TR) {
         const int r = row0 + tid;
         int e = 0;
-        if (STEP == 1) e = r < A.n ? A.rexp[b * A.kp + r] : 0;
-        else if (r < nrows) {
+        if (r < nrows) {
             e = A.rexp[b * A.mp + r];
             if (e == INT_MIN) e = 0;
         }
@@ -431,14 +399,13 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void k_split_gemm(SplitArgs A) {
     }
     if (!active) return;
 
-    // ---- epilogue: FP64 recombination and stores
+    // ---- epilogue: FP64 recombination and stores into the packed output
     typedef __attribute__((address_space(1))) double gdouble;
     gdouble* __restrict__ dst = (gdouble*)A.dst;
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
         const int col = cw + 16 * cb + lm;
         const int fc = col < A.mp ? A.cexp[col] : 0;
-        int emax = INT_MIN;
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb) {
 #pragma unroll
@@ -450,47 +417,237 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void k_split_gemm(SplitArgs A) {
                 for (int g = S - 1; g >= 0; --g)
                     v += ldexp((double)acc[g][rb][cb][q], base - 8 * (g + 2));
                 const int r = row0 + il;
-                if (STEP == 1) {
-                    if (r < A.n && col < ncols) {
-                        dst[(b * A.mp + col) * (int64_t)A.kp + r] = v;
-                        if (v != 0.0) emax = max(emax, exp_above(v));
-                    }
-                } else {
-                    if (r < nrows && col <= r) {
-                        const int64_t p = pair0(r, col);
-                        if (p < limit) {
-                            if (A.store_t) dst[p * A.nb + b] = v;
-                            else dst[dst_off + p] = v;
-                        }
+                if (r < nrows && col <= r) {
+                    const int64_t p = pair0(r, col);
+                    if (p < limit) {
+                        if (A.store_t) dst[p * A.nb + b] = v;
+                        else dst[dst_off + p] = v;
                     }
                 }
             }
         }
-        if (STEP == 1) {
-            // row exponent of U row (b, col): max over the 32 rows of this tile
-            emax = max(emax, __shfl_xor(emax, 16));
-            emax = max(emax, __shfl_xor(emax, 32));
-            if (lg == 0 && col < ncols && emax != INT_MIN) atomicMax(&A.dexp[b * A.mp + col], emax);
-        }
     }
 }
 
-template <int S, int STEP>
-hipError_t launch_split_gemm(const SplitArgs& a, hipStream_t st) {
-    constexpr int NW = STEP == 1 ? NW1 : NW2;
+// Step 1 as one work-group per matrix b: the row tiles of Z_b run as one pipeline of
+// K stages (tile t, stage k is global stage t * nst + k), so the A gathers of the next
+// tile are in flight while a tile's epilogue runs, and the work-group starts only once
+// per matrix.  Each wave owns 32 columns over all rows of b, so the exponent of U row
+// (b, col), the maximum over every row tile, is reduced in registers and written once.
+// Row exponents of the A rows are loaded one tile ahead into registers: the committing
+// thread's row (rc0 / rc1 by tile parity) and the 8 rows of the lane's output elements.
+template <int S>
+__global__ __launch_bounds__(NW1 * 64, 1) void k_split_rows(SplitArgs A) {
+    constexpr int NTHR = NW1 * 64;
+    constexpr int EPT = TR * KCH / NTHR;
+    static_assert(EPT == 2, "two A elements per thread and chunk");
+    __shared__ _Float16 As[2][2][S][TR][APAD];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int lm = lane & 15, lg = lane >> 4;
+    const int64_t b = xcd_item(blockIdx.x);
+    if (b >= A.nitems) return;
+    const int kc = A.kp / KCH, nst = kc / 2;
+    const int ntr = (A.n + TR - 1) / TR;
+    const int ntot = ntr * nst;                                    // stages of the matrix
+    const int ncols = A.half == 1 ? A.m : (int)pair_row(b) + 1;   // half 2: columns j <= l
+    const int cw = 32 * w;
+    const bool active = cw < ncols;
+
+    const int si = tid / (KCH / EPT), sk = EPT * (tid % (KCH / EPT));
+    gcdouble* __restrict__ src = (gcdouble*)A.src + b * A.src_ld;
+    const int* __restrict__ rexp = A.rexp + b * A.kp;
+    // A element u of chunk ch of global stage g: row t * TR + si of Z_b, column s.
+    auto elem = [&](int u, int g, int ch, int64_t& idx) -> bool {
+        const int t = g / nst;
+        const int r = t * TR + si, s = (g - t * nst) * KST + ch * KCH + sk + u;
+        const bool ok = r < A.n && s < A.n;
+        const int hi = r >= s ? r : s, lo = r >= s ? s : r;
+        idx = ok ? pair0(hi, lo) : 0;
+        return ok;
+    };
+    auto fetch = [&](int g, double (&xs)[2 * EPT]) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int u = 0; u < EPT; ++u) {
+                int64_t idx;
+                (void)elem(u, g, c, idx);
+                xs[EPT * c + u] = src[idx];
+            }
+    };
+    auto commit = [&](int g, int buf, int e, const double (&xs)[2 * EPT]) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            int64_t idx;
+            const double v0 = elem(0, g, c, idx) ? xs[EPT * c] : 0.0;
+            const double v1 = elem(1, g, c, idx) ? xs[EPT * c + 1] : 0.0;
+            uint32_t wd[S];
+            digits_pair<S>(v0, v1, e, e, wd);
+#pragma unroll
+            for (int s = 0; s < S; ++s) *(uint32_t*)&As[buf][c][s][si][sk] = wd[s];
+        }
+    };
+    auto row_exp = [&](int r) { return r < A.n ? rexp[r] : 0; };
+
+    f32x4 acc[S][2][2];
+    auto clear = [&]() {
+#pragma unroll
+        for (int g = 0; g < S; ++g)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[g][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    };
+    clear();
+
+    f16x8 bcur[S][2], bnext[S][2];
+    const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)A.cs, (short)0, (int)(sizeof(_Float16) * (size_t)S * A.kp * A.mp), 0x00020000);
+    unsigned boff[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+        boff[cb] = (unsigned)(min(cw + 16 * cb + lm, A.mp - 1) * KCH + 8 * lg) * 2u;
+    auto load_b = [&](int ch, f16x8 (&bf)[S][2]) {
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const unsigned so = (unsigned)(((j * kc + ch) * A.mp) * KCH) * 2u;
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+                bf[j][cb] = __builtin_bit_cast(
+                    f16x8, __builtin_amdgcn_raw_buffer_load_b128(crs, boff[cb], so, 0));
+        }
+    };
+    auto mfma_chunk = [&](int buf, int c, const f16x8 (&bc)[S][2]) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            f16x8 af[2];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+                af[rb] = *(const f16x8*)&As[buf][c][i][16 * rb + lm][8 * lg];
+#pragma unroll
+            for (int j = 0; i + j < S; ++j)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc[i + j][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                            af[rb], bc[j][cb], acc[i + j][rb][cb], 0, 0, 0);
+        }
+    };
+
+    int fc[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        const int col = cw + 16 * cb + lm;
+        fc[cb] = col < A.mp ? A.cexp[col] : 0;
+    }
+    int emax[2] = {INT_MIN, INT_MIN};
+    typedef __attribute__((address_space(1))) double gdouble;
+    gdouble* __restrict__ dst = (gdouble*)A.dst;
+
+    double xa[2 * EPT], xb[2 * EPT];
+    load_b(0, bcur);
+    fetch(0, xa);
+    fetch(ntot > 1 ? 1 : 0, xb);
+    int rc0 = row_exp(si), rc1 = row_exp(TR + si);                 // committing rows, tiles 0, 1
+    int er[2][4];                                                  // output rows of this tile
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) er[rb][q] = row_exp(16 * rb + 4 * lg + q);
+    commit(0, 0, rc0, xa);
+    __syncthreads();
+    for (int g = 0; g < ntot; ++g) {
+        const int buf = g & 1;
+        const int t = g / nst, k = g - t * nst;
+        load_b(2 * k + 1, bnext);
+        __builtin_amdgcn_sched_barrier(0);
+        if (active) mfma_chunk(buf, 0, bcur);
+        __builtin_amdgcn_sched_barrier(0);
+        load_b(2 * k + 2 < kc ? 2 * k + 2 : 0, bcur);
+        fetch(min(g + 2, ntot - 1), xa);
+        __builtin_amdgcn_sched_barrier(0);
+        if (active) mfma_chunk(buf, 1, bnext);
+        __builtin_amdgcn_sched_barrier(0);
+        if (g + 1 < ntot) {
+            const int t1 = (g + 1) / nst;
+            commit(g + 1, buf ^ 1, (t1 & 1) ? rc1 : rc0, xb);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 2 * EPT; ++u) xb[u] = xa[u];
+        if (k == nst - 1) {
+            // ---- tile t done: FP64 recombination into U rows (b, col), element r
+            const int row0 = t * TR;
+            if (active) {
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) {
+                    const int col = cw + 16 * cb + lm;
+#pragma unroll
+                    for (int rb = 0; rb < 2; ++rb) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int il = 16 * rb + 4 * lg + q;
+                            const int base = er[rb][q] + fc[cb] + 2;
+                            double v = 0.0;
+#pragma unroll
+                            for (int gg = S - 1; gg >= 0; --gg)
+                                v += ldexp((double)acc[gg][rb][cb][q], base - 8 * (gg + 2));
+                            const int r = row0 + il;
+                            if (r < A.n && col < ncols) {
+                                dst[(b * A.mp + col) * (int64_t)A.kp + r] = v;
+                                if (v != 0.0) emax[cb] = max(emax[cb], exp_above(v));
+                            }
+                        }
+                    }
+                }
+            }
+            clear();
+            // exponents of the tile after next (committing rows) and of the next tile
+            // (output rows)
+            if (t & 1) rc1 = row_exp((t + 2) * TR + si);
+            else rc0 = row_exp((t + 2) * TR + si);
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) er[rb][q] = row_exp((t + 1) * TR + 16 * rb + 4 * lg + q);
+        }
+    }
+    // U row exponents: maximum over the four row groups of the wave's lanes
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        int e = emax[cb];
+        e = max(e, __shfl_xor(e, 16));
+        e = max(e, __shfl_xor(e, 32));
+        const int col = cw + 16 * cb + lm;
+        if (lg == 0 && col < ncols) A.dexp[b * A.mp + col] = e;
+    }
+}
+
+template <int S>
+hipError_t launch_split_rows(const SplitArgs& a, hipStream_t st) {
     const int64_t g = xcd_grid(a.nitems);
     if (g <= 0 || g > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_split_gemm<S, STEP, NW>), dim3((unsigned)g), dim3(NW * 64), 0, st, a);
+    hipLaunchKernelGGL((k_split_rows<S>), dim3((unsigned)g), dim3(NW1 * 64), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int S>
+hipError_t launch_split_tri(const SplitArgs& a, hipStream_t st) {
+    const int64_t g = xcd_grid(a.nitems);
+    if (g <= 0 || g > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_split_tri<S>), dim3((unsigned)g), dim3(NW2 * 64), 0, st, a);
     return hipGetLastError();
 }
 
 template <int STEP>
 hipError_t launch_split_s(int s, const SplitArgs& a, hipStream_t st) {
     switch (s) {
-        case 3: return launch_split_gemm<3, STEP>(a, st);
-        case 4: return launch_split_gemm<4, STEP>(a, st);
-        case 5: return launch_split_gemm<5, STEP>(a, st);
-        case 6: return launch_split_gemm<6, STEP>(a, st);
+        case 3: return STEP == 1 ? launch_split_rows<3>(a, st) : launch_split_tri<3>(a, st);
+        case 4: return STEP == 1 ? launch_split_rows<4>(a, st) : launch_split_tri<4>(a, st);
+        case 5: return STEP == 1 ? launch_split_rows<5>(a, st) : launch_split_tri<5>(a, st);
+        case 6: return STEP == 1 ? launch_split_rows<6>(a, st) : launch_split_tri<6>(a, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -574,7 +731,6 @@ int run_half_split(cipsi_ijkl_ctx* ctx, const HalfArgs& a, int, int n_slices) {
         return CIPSI_ERR_HIP;
     hipLaunchKernelGGL(k_split_rowexp, dim3((unsigned)a.nb), dim3(256), 0, st, a.src, a.src_ld, n,
                        kp, rexp);
-    hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(256), 0, st, uexp, a.nb * (int64_t)mp, INT_MIN);
     if (hipGetLastError() != hipSuccess) return CIPSI_ERR_HIP;
 
     SplitArgs x{};
@@ -592,7 +748,7 @@ int run_half_split(cipsi_ijkl_ctx* ctx, const HalfArgs& a, int, int n_slices) {
     x.rexp = rexp;
     x.dst = u;
     x.dexp = uexp;
-    x.nitems = a.nb * (int64_t)((n + TR - 1) / TR) * (MAXC / (NW1 * 32));
+    x.nitems = a.nb;
     if ((e = launch_split_s<1>(s, x, st)) != hipSuccess) return CIPSI_ERR_HIP;
     // step 2: O_b^T = U_b C, packed triangle
     x.src = u;
