@@ -154,7 +154,7 @@ def split_roofline(n, m, run, args):
     fl = split_flops_executed(n, m, run["slices"])
     t = run["stages"]["half1"] + run["stages"]["half2"]
     achieved = fl / (t * 1e-3) / 1e12 if t else 0.0
-    return {"bound": "mfma", "kernel": "k_split_gemm (both halves, executed FP16 MFMA flops)",
+    return {"bound": "mfma", "kernel": "k_split_rows + k_split_tri (both halves, executed FP16 MFMA flops)",
             "achieved": round(achieved, 3), "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP16_PEAK_TFLOPS, 4), "traffic": None,
             "stage_ms": {k: round(v, 4) for k, v in run["stages"].items()}}
