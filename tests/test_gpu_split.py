@@ -49,6 +49,23 @@ def test_split_default_slices_within_tolerance(ctx, n, m, rank, decay, seed, fro
     assert err <= TOL
 
 
+@pytest.mark.parametrize("n,m", [(256, 256), (256, 1), (33, 1), (130, 129), (17, 16), (200, 144)])
+def test_split_edge_sizes(ctx, n, m):
+    """The largest basis (N_AO = N_MO = 256: every 32-row tile and 128-column tile full,
+    K = 256), a single MO, a basis one row past a 32-row tile, MO counts that end in the
+    first rows of the second 128-column tile of the triangle (half-2 rows l >= 128), and a
+    basis below one 32-row tile: <= 1e-10 against the FP64 path."""
+    print(f"[split edge] n={n} m={m}", flush=True)
+    ao, c = inputs(n, m, 2 * n, 10.0, 3000 + n + m)
+    ref = ctx.transform(ao, c)
+    out = ctx.transform(ao, c, precision="split_fp16")
+    err = float((out - ref).abs().max())
+    print(f"[split edge] n={n} m={m} maxabs={err:.3e} vmax={float(ref.abs().max()):.3e}",
+          flush=True)
+    assert err <= TOL
+    assert float(ref.abs().max()) > 0.0
+
+
 def test_split_error_falls_with_slices(ctx):
     ao, c = inputs(40, 36, 120, 8.0, 2024)
     ref = ctx.transform(ao, c)
