@@ -47,10 +47,10 @@ constexpr int KST = 2 * KCH;  // K per stage: the chunks staged between two barr
 constexpr int APAD = 40;      // LDS row stride of a digit plane, in halves
 constexpr int MAXC = 256;     // output columns of a matrix (N_MO rounded up to 16)
 // Waves per work-group: each wave owns 32 output columns.  Step 1 runs one 8-wave
-// work-group per CU over all columns of a row tile (the A digits in LDS serve 8 waves);
-// step 2, whose triangle leaves many columns empty, runs two 4-wave work-groups per CU
-// over column tiles of 128, so idle column tiles cost nothing and one work-group's
-// barriers and epilogue overlap the other's MFMAs.
+// work-group per matrix over all its columns (the A digits in LDS serve 8 waves); step 2,
+// whose triangle leaves many columns empty, runs two 4-wave work-groups per CU over
+// column tiles of 128, so idle column tiles cost nothing and one work-group's barriers
+// and epilogue overlap the other's MFMAs.
 constexpr int NW1 = 8;
 constexpr int NW2 = 4;
 constexpr int TC = NW2 * 32;  // step-2 column tile
@@ -210,9 +210,9 @@ struct SplitArgs {
 
 // Step 2: one work item is TR rows c of U_b times the TC columns a of column tile ct,
 // for the (rt, ct) pairs of the triangle a <= c (tri_items per matrix).  Half 1 takes
-// consecutive matrices b for one pair (H^T stores: an XCD's chunk of items fills whole
-// 128-byte lines) or consecutive pairs of one b; half 2 finds the MO pair row l in the
-// prefix table.  Two 4-wave work-groups run per CU.
+// consecutive matrices b for one pair (H^T stores: neighbouring items, which run on one
+// XCD, write neighbouring doubles of each H^T row) or consecutive pairs of one b; half 2
+// finds the MO pair row l in the prefix table.  Two 4-wave work-groups run per CU.
 template <int S>
 __global__ __launch_bounds__(NW2 * 64, 2) void k_split_tri(SplitArgs A) {
     constexpr int NTHR = NW2 * 64;
