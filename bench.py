@@ -44,11 +44,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MO integrals/s at N_AO=240: FP64 MFMA vs split-FP16 at <=1e-10 abs err"
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE runs of this workload (fetched = 2.0 x FETCH_SIZE, written = 1.0 x
-# WRITE_SIZE, factors from a 1 GiB copy in the same process); keyed by
-# (n_ao, n_mo, precision).  Summaries: profiles/fp64_n240_kernels.md.
-MEASURED_TRAFFIC = {(240, 240, "fp64"): (2.0 * 3631400.0 + 6877300.0) * 1024}
+# HBM bytes from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload
+# (fetched = 2.0 x FETCH_SIZE, written = 1.0 x WRITE_SIZE, factors from a 1 GiB copy in
+# the same process); keyed by (n_ao, n_mo, precision).  FP64: one launch of half 1 (H^T
+# stores), KiB averaged over four launches.  Split: the split kernels of both halves of one
+# transform (row exponents, step 1, step 2).  Summaries: profiles/fp64_n240_kernels.md,
+# profiles/split_fp16_n240_kernels.md.
+MEASURED_TRAFFIC = {(240, 240, "fp64"): (2.0 * 4058352.0 + 14190385.0) * 1024,
+                    (240, 240, "split_fp16"): (2.0 * 34947414.0 + 34269310.0) * 1024}
 # k_gather over 2^27 uniformly random labels at N_MO = 240, per launch: 2.0 x FETCH_SIZE
 # 8.6412e6 KiB + 1.0 x WRITE_SIZE 1.0486e6 KiB (profiles/consumers_n240.md).
 MEASURED_TRAFFIC_GATHER = {(240, 1 << 27): (2.0 * 8.6412e6 + 1.0486e6) * 1024}
@@ -156,7 +159,8 @@ def split_roofline(n, m, run, args):
     achieved = fl / (t * 1e-3) / 1e12 if t else 0.0
     return {"bound": "mfma", "kernel": "k_split_rows + k_split_tri (both halves, executed FP16 MFMA flops)",
             "achieved": round(achieved, 3), "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP16_PEAK_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved / FP16_PEAK_TFLOPS, 4),
+            "traffic": MEASURED_TRAFFIC.get((n, m, "split_fp16")) if run["slices"] == 6 else None,
             "stage_ms": {k: round(v, 4) for k, v in run["stages"].items()}}
 
 
