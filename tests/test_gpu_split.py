@@ -145,6 +145,41 @@ def test_split_host_entry_error_estimate(ctx):
     assert err <= est <= 1e-6
 
 
+def test_split_under_graph_capture(ctx):
+    """The split device transform launches only kernels on the caller's stream once its
+    workspace and triangle tables exist, so a caller can capture it: each replay, with new
+    inputs copied in and eager work in between, equals the eager split result bit for
+    bit (both halves, step 1 and step 2, are deterministic)."""
+    n, m = 72, 66
+    print(f"[split graph] n={n} m={m}", flush=True)
+    cases = [inputs(n, m, 2 * n, 6.0, 720 + k) for k in range(3)]
+    V = cases[0][0].clone()
+    c = cases[0][1].clone()
+    refs = [ctx.transform(a, b, precision="split_fp16") for a, b in cases]
+    fp64 = ctx.transform(cases[0][0], cases[0][1])
+    torch.cuda.synchronize()
+    assert float((refs[0] - fp64).abs().max()) <= TOL
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):                 # warm-up: workspace and tables exist
+        ctx.transform(V, c, precision="split_fp16")
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = ctx.transform(V, c, precision="split_fp16")
+    for rep in range(2):
+        for k, (a, b) in enumerate(cases):
+            V.copy_(a)
+            c.copy_(b)
+            scratch = torch.randn(1 << 20, device="cuda", dtype=torch.float64)
+            scratch.mul_(2.0)                  # eager work between replays
+            out.fill_(float("nan"))
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(out, refs[k]), f"replay {rep} case {k}"
+
+
 def test_split_rejects_bad_slice_counts(ctx):
     from cipsi_amd import ijkl
     ao, c = inputs(12, 10, 24, 3.0, 3)
