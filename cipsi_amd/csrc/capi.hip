@@ -3,11 +3,12 @@
 // The pipeline for one C1 transform (all on one HIP stream, device-resident):
 //   pad_c        C -> zero-padded copy in 16-column blocks used by the MFMA tiles
 //   expand       packed AO block -> full symmetric pair matrix (rows contiguous)
-//   half 1       per AO pair: C^T X C lower triangle   (tpqkld.f)      -> H[P_A][P_M]
-//   transpose    H -> H^T (MO-pair rows)               (replaces unit 40, itijkl.f:249-278)
+//   half 1       per AO pair: C^T X C lower triangle   (tpqkld.f)      -> H^T (MO-pair
+//                rows; replaces unit 40, itijkl.f:249-278)
 //   half 2       per MO pair: C^T Y C, ij <= kl        (tijkld.f)      -> packed MO stream
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -306,7 +307,9 @@ int cipsi_ijkl_destroy(cipsi_ijkl_ctx* ctx) {
         e = nullptr;
     }
     if (ctx->front_ev) (void)hipEventDestroy(ctx->front_ev);
+    if (ctx->entry_ev) (void)hipEventDestroy(ctx->entry_ev);
     if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+    if (ctx->up_stream) (void)hipStreamDestroy(ctx->up_stream);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return CIPSI_OK;
@@ -341,7 +344,7 @@ int cipsi_ijkl_synchronize(cipsi_ijkl_ctx* ctx) {
 // transpose.  The split path's last step of half 1 stores H^T the same way.
 static int transform_front(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* d_ao,
                            const double* d_c, int ldc, int precision, int n_slices,
-                           const double** ht, double** cr, bool expanded = false) {
+                           const double** ht, double** cr) {
     const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
     int rc;
     if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
@@ -349,7 +352,7 @@ static int transform_front(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double
     if ((rc = run_pad_c(ctx, n_ao, n_mo, d_c, ldc, cr))) return rc;
     double* vsq = (double*)ctx->vsq.ptr;
     double* h = (double*)ctx->h.ptr;
-    if (!expanded) {
+    {
         StageTimer tm(ctx, "expand");
         HIP_TRY(ctx, launch_expand_packed(d_ao, pa, vsq, ctx->stream));
     }
@@ -681,14 +684,19 @@ static int half2_stream_out(cipsi_ijkl_ctx* ctx, int n, int m, const double* ht,
             HIP_TRY(ctx, hipEventRecord(ctx->seg_ev[s], ctx->stream));
         }
     }
+    // copy chunks: a slot each, doubling from 8 MiB over the first ones, so the copy starts
+    // after the first segment
     constexpr int S = cipsi_ijkl_ctx::kPinSlots;
-    const size_t chunk = cipsi_ijkl_ctx::kPinBytes;
     const size_t bytes = sizeof(double) * (size_t)nout;
-    const int64_t nch = (int64_t)((bytes + chunk - 1) / chunk);
+    std::vector<size_t> cut(1, 0);
+    for (size_t len = size_t(8) << 20; cut.back() < bytes;
+         len = std::min(cipsi_ijkl_ctx::kPinBytes, 2 * len))
+        cut.push_back(std::min(bytes, cut.back() + len));
+    const int64_t nch = (int64_t)cut.size() - 1;
     int waited = -1;                               // last segment the copy stream waits for
     auto issue = [&](int64_t k) -> int {
         const int s = (int)(k % S);
-        const size_t off = (size_t)k * chunk, len = std::min(chunk, bytes - off);
+        const size_t off = cut[k], len = cut[k + 1] - cut[k];
         int r = pin_wait(ctx, s);
         if (r) return r;
         const int64_t end = (int64_t)((off + len) / sizeof(double));
@@ -709,68 +717,125 @@ static int half2_stream_out(cipsi_ijkl_ctx* ctx, int n, int m, const double* ht,
     touch->join();
     for (int64_t k = 0; k < nch; ++k) {
         const int s = (int)(k % S);
-        const size_t off = (size_t)k * chunk, len = std::min(chunk, bytes - off);
         if ((rc = pin_wait(ctx, s))) return rc;
-        parallel_memcpy((char*)mo + off, ctx->pin[s], len);
+        parallel_memcpy((char*)mo + cut[k], ctx->pin[s], cut[k + 1] - cut[k]);
         if (k + S < nch && (rc = issue(k + S))) return rc;
     }
     return CIPSI_OK;
 }
 
-// The upload of the packed AO block for the FP64 host entry point, with the expand run
-// behind it: every chunk goes through the pinned slots on the context stream, and as soon
-// as a chunk completes the packed rows of some expand tile rows, those tile rows are
-// expanded on the copy stream after the chunk's event.  front_ev marks the last of them;
-// the caller makes the context stream wait for it before half 1.
-static int upload_expand(cipsi_ijkl_ctx* ctx, double* d_in, const double* ao, int64_t pa,
-                         double* vsq) {
+// The first packed row that starts at or after element e: the least a with npairs(a) >= e.
+static int64_t first_row_from(int64_t e) {
+    int64_t a = (int64_t)((std::sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+    while (a > 0 && npairs(a - 1) >= e) --a;
+    while (npairs(a) < e) ++a;
+    return a;
+}
+
+// The front of the FP64 packed host entry point: the upload of the packed AO block with
+// the expand and half 1 run behind it.  Row b of the pair matrix is packed row b (columns
+// s <= b) and column b of the packed rows after it (s > b), so it is complete once packed
+// rows b .. P_A - 1 are on the device.  The block is therefore sent from its end, chunk by
+// chunk through the pinned slots on the upload stream.  When a chunk completes the packed
+// rows of more expand tile rows, those are expanded on the copy stream after the chunk's
+// event, and half 1 runs on the context stream over the pair-matrix rows that every tile
+// row from there to the end now covers.  Chunks shrink towards the start of the block,
+// where one holds many short rows, so the last chunks complete few rows each and little
+// of half 1 remains when the upload ends (at N = 240 the upload takes about 58 ms and
+// half 1 about 25 ms).  The context stream's earlier work is ordered before the upload.
+// On return every launch is queued; the upload stream holds the last copy.
+static int upload_front(cipsi_ijkl_ctx* ctx, int n, int m, double* d_in, const double* ao,
+                        double* vsq, const double* cr, double* ht) {
+    const int var = ctx_half_variant(ctx, 1, n, m, CIPSI_PREC_FP64);
+    if (!variant_supports(var, row_class(round16(n)))) return CIPSI_ERR_UNSUPPORTED;
     int rc = pin_ready(ctx);
     if (rc) return rc;
     if (!ctx->copy_stream)
         HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    if (!ctx->up_stream)
+        HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->up_stream, hipStreamNonBlocking));
     if (!ctx->front_ev) HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->front_ev, hipEventDisableTiming));
-    const size_t chunk = cipsi_ijkl_ctx::kPinBytes;
-    const size_t bytes = sizeof(double) * (size_t)npairs(pa);
+    if (!ctx->entry_ev) HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->entry_ev, hipEventDisableTiming));
+    HIP_TRY(ctx, hipEventRecord(ctx->entry_ev, ctx->stream));
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->up_stream, ctx->entry_ev, 0));
+    const int64_t pa = npairs(n), pm = npairs(m);
+    const int lt = variant_columns(var);
+    const int64_t nt = (m + lt - 1) / lt;              // half-1 work items per matrix
+    HalfArgs a{};
+    a.src = vsq;
+    a.src_ld = pa;
+    a.cr = cr;
+    a.ldm = round16(m);
+    a.dst = ht;
+    a.n = n;
+    a.m = m;
+    a.np = round16(n);
+    a.mp = round16(m);
+    a.half = 1;
+    a.nb = pa;
+    a.b_begin = 0;
+    a.dst_ld = pm;
+    a.store_t = 1;
+    a.tile_start = nullptr;
+    // chunk lengths in elements: about a quarter of what is left, between kMinChunk and a
+    // slot, and doubling from kFirstChunk over the first chunks (the DMA engine starts
+    // after a short host copy); a chunk starts on a 4 KiB boundary (rounded up, so it
+    // never exceeds a slot)
+    constexpr int64_t kSlot = (int64_t)(cipsi_ijkl_ctx::kPinBytes / sizeof(double));
+    constexpr int64_t kMinChunk = int64_t(1) << 19;
+    constexpr int64_t kFirstChunk = int64_t(1) << 20;
+    constexpr int64_t kAlign = 512;
+    // half 1 is launched once its next range holds this many matrices (or at the end):
+    // about two rounds of work-groups over 256 CUs
+    constexpr int64_t kMinRows = 256;
     const int64_t ntr = (pa + kExpandTile - 1) / kExpandTile;
-    int64_t done = 0;                                  // tile rows expanded so far
-    int s = 0;
-    for (size_t off = 0; off < bytes; off += chunk, s = (s + 1) % cipsi_ijkl_ctx::kPinSlots) {
-        const size_t len = std::min(chunk, bytes - off);
+    int64_t t_done = ntr;                              // tile rows [t_done, ntr) expanded
+    int64_t b_done = pa;                               // half 1 queued for matrices >= b_done
+    int64_t hi = npairs(pa);
+    int64_t ramp = kFirstChunk;
+    for (int s = 0; hi > 0; s = (s + 1) % cipsi_ijkl_ctx::kPinSlots) {
+        const int64_t want = std::min({kSlot, ramp, std::max(kMinChunk, hi / 4)});
+        ramp = std::min(kSlot, 2 * ramp);
+        const int64_t lo = hi <= want ? 0 : (hi - want + kAlign - 1) / kAlign * kAlign;
+        const size_t off = sizeof(double) * (size_t)lo, len = sizeof(double) * (size_t)(hi - lo);
         if ((rc = pin_wait(ctx, s))) return rc;
         parallel_memcpy(ctx->pin[s], (const char*)ao + off, len);
         HIP_TRY(ctx, hipMemcpyAsync((char*)d_in + off, ctx->pin[s], len, hipMemcpyHostToDevice,
-                                    ctx->stream));
-        HIP_TRY(ctx, hipEventRecord(ctx->pin_ev[s], ctx->stream));
+                                    ctx->up_stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->pin_ev[s], ctx->up_stream));
         ctx->pin_busy[s] = true;
-        // packed rows a with npairs(a + 1) <= elements uploaded are complete
-        const int64_t have = (int64_t)((off + len) / sizeof(double));
-        int64_t rows = (int64_t)((std::sqrt(8.0 * (double)have + 1.0) - 1.0) * 0.5);
-        while (rows > 0 && npairs(rows) > have) --rows;
-        while (npairs(rows + 1) <= have) ++rows;
-        const int64_t tr = off + len == bytes ? ntr : std::min(ntr, rows / kExpandTile);
-        if (tr > done) {
+        hi = lo;
+        // packed rows >= first_row_from(lo) have arrived: tile rows from t_ready on
+        const int64_t t_ready = std::min(ntr, (first_row_from(lo) + kExpandTile - 1) / kExpandTile);
+        if (t_ready < t_done) {
             HIP_TRY(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->pin_ev[s], 0));
-            HIP_TRY(ctx, launch_expand_rows(d_in, pa, vsq, done, tr, ctx->copy_stream));
-            done = tr;
+            HIP_TRY(ctx, launch_expand_rows(d_in, pa, vsq, t_ready, t_done, ctx->copy_stream));
+            t_done = t_ready;
+            const int64_t b_ready = std::min(pa, t_ready * kExpandTile);
+            if (b_done - b_ready >= kMinRows || b_ready == 0) {
+                HIP_TRY(ctx, hipEventRecord(ctx->front_ev, ctx->copy_stream));
+                HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->front_ev, 0));
+                a.item_begin = b_ready * nt;
+                a.ntile_total = b_done * nt;
+                HIP_TRY(ctx, launch_half_fp64(a, var, ctx->stream));
+                b_done = b_ready;
+            }
         }
     }
-    HIP_TRY(ctx, hipEventRecord(ctx->front_ev, ctx->copy_stream));
-    return CIPSI_OK;
+    return CIPSI_OK;                                   // the last chunk (lo = 0) queued the rest
 }
 
 // The device part of a host entry point from the resident packed block on: FP64 streams
 // its output to the caller during half 2, the split path copies it after the transform.
-// expanded: the pair matrix is already in vsq (upload_expand).
 static int transform_to_host(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const double* d_ao,
                              const double* d_c, int ldc, int precision, int n_slices,
-                             double* mo, bool expanded, PageToucher* touch) {
+                             double* mo, PageToucher* touch) {
     double* d_mo = (double*)ctx->io_out.ptr;
     int rc;
     if (precision == CIPSI_PREC_FP64) {
         const double* ht;
         double* cr;
-        if ((rc = transform_front(ctx, n_ao, n_mo, d_ao, d_c, ldc, precision, n_slices, &ht, &cr,
-                                  expanded)))
+        if ((rc = transform_front(ctx, n_ao, n_mo, d_ao, d_c, ldc, precision, n_slices, &ht, &cr)))
             return rc;
         return half2_stream_out(ctx, n_ao, n_mo, ht, cr, d_mo, mo, touch);
     }
@@ -813,18 +878,18 @@ static int upload_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* c, int ldc,
 double err_estimate(int n, int precision, int n_slices, double vmax);
 
 // Scan the device copy of the caller's values (finiteness, max |x|; up to the first zero
-// label when labels are given) and wait for the result, so a non-finite input is
-// rejected before any transform work.  The scan runs at HBM speed instead of one host
-// core's.
+// label when labels are given) on stream st, after the upload, and wait for the result.
+// A non-finite input is rejected before any output reaches the caller.  The scan runs at
+// HBM speed instead of one host core's.
 static int scan_uploaded(cipsi_ijkl_ctx* ctx, const double* d_p, int64_t n,
-                         const int32_t* d_labels, double* vmax) {
+                         const int32_t* d_labels, double* vmax, hipStream_t st) {
     int rc = ctx_alloc(ctx, ctx->scan, 64);
     if (rc) return rc;
     unsigned long long* s = (unsigned long long*)ctx->scan.ptr;
-    HIP_TRY(ctx, launch_scan_input(d_p, n, d_labels, s, ctx->stream));
+    HIP_TRY(ctx, launch_scan_input(d_p, n, d_labels, s, st));
     unsigned long long hs[3] = {0ull, 0ull, 0ull};
-    HIP_TRY(ctx, hipMemcpyAsync(hs, s, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(hs, s, sizeof(hs), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
     if (hs[1]) return CIPSI_ERR_NONFINITE;
     double v;
     std::memcpy(&v, &hs[0], sizeof(v));
@@ -847,26 +912,36 @@ int cipsi_ijkl_transform_packed(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
     if ((rc = upload_c(ctx, n_ao, n_mo, c, ldc, &d_c))) return rc;
     PageToucher touch(mo, sizeof(double) * nout, ao, sizeof(double) * nin, c,
                       sizeof(double) * (size_t)ldc * n_mo);
-    // FP64: the expand runs during the upload (the scan below still rejects a non-finite
-    // block before half 1 starts)
-    const bool fp64 = precision == CIPSI_PREC_FP64;
-    if (fp64) {
-        const int64_t pa = npairs(n_ao);
-        if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
-        if ((rc = upload_expand(ctx, (double*)ctx->io_in.ptr, ao, pa, (double*)ctx->vsq.ptr)))
-            return rc;
-    } else if ((rc = ctx_copy_h2d(ctx, ctx->io_in.ptr, ao, sizeof(double) * nin))) {
-        return rc;
-    }
     double vmax = 0.0;
-    if ((rc = scan_uploaded(ctx, (const double*)ctx->io_in.ptr, nin, nullptr, &vmax))) {
-        if (fp64) (void)hipStreamSynchronize(ctx->copy_stream);
-        return rc;
+    if (precision == CIPSI_PREC_FP64) {
+        // the expand and half 1 run during the upload; the scan of the whole block still
+        // rejects a non-finite one before half 2 and any copy to the caller
+        const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
+        double* cr;
+        if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
+        if ((rc = ctx_alloc(ctx, ctx->h, sizeof(double) * pa * pm))) return rc;
+        if ((rc = run_pad_c(ctx, n_ao, n_mo, d_c, ldc, &cr))) return rc;
+        if ((rc = upload_front(ctx, n_ao, n_mo, (double*)ctx->io_in.ptr, ao, (double*)ctx->vsq.ptr,
+                               cr, (double*)ctx->h.ptr)))
+            return rc;
+        if ((rc = scan_uploaded(ctx, (const double*)ctx->io_in.ptr, nin, nullptr, &vmax,
+                                ctx->up_stream))) {
+            (void)hipStreamSynchronize(ctx->copy_stream);
+            (void)hipStreamSynchronize(ctx->stream);
+            return rc;
+        }
+        if ((rc = half2_stream_out(ctx, n_ao, n_mo, (const double*)ctx->h.ptr, cr,
+                                   (double*)ctx->io_out.ptr, mo, &touch)))
+            return rc;
+    } else {
+        if ((rc = ctx_copy_h2d(ctx, ctx->io_in.ptr, ao, sizeof(double) * nin))) return rc;
+        if ((rc = scan_uploaded(ctx, (const double*)ctx->io_in.ptr, nin, nullptr, &vmax,
+                                ctx->stream)))
+            return rc;
+        if ((rc = transform_to_host(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr, d_c, ldc,
+                                    precision, n_slices, mo, &touch)))
+            return rc;
     }
-    if (fp64) HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->front_ev, 0));
-    if ((rc = transform_to_host(ctx, n_ao, n_mo, (const double*)ctx->io_in.ptr, d_c, ldc,
-                                precision, n_slices, mo, fp64, &touch)))
-        return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (err_est) *err_est = err_estimate(n_ao, precision, n_slices, vmax);
     return CIPSI_OK;
@@ -896,7 +971,7 @@ int cipsi_ijkl_transform_labels(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
     }
     double vmax = 0.0;          // 8 max |xx| before the first zero label (transd's largest scale)
     if ((rc = scan_uploaded(ctx, (const double*)ctx->io_in.ptr, n_entries,
-                            (const int32_t*)ctx->io_in2.ptr, &vmax)))
+                            (const int32_t*)ctx->io_in2.ptr, &vmax, ctx->stream)))
         return rc;
     vmax *= 8.0;
     if ((rc = ctx_alloc(ctx, ctx->packed, sizeof(double) * npairs(npairs(n_ao))))) return rc;
@@ -905,7 +980,7 @@ int cipsi_ijkl_transform_labels(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, const d
                                            (double*)ctx->packed.ptr)))
         return rc;
     if ((rc = transform_to_host(ctx, n_ao, n_mo, (const double*)ctx->packed.ptr, d_c, ldc,
-                                precision, n_slices, mo, false, &touch)))
+                                precision, n_slices, mo, &touch)))
         return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (err_est) *err_est = err_estimate(n_ao, precision, n_slices, vmax);

@@ -43,7 +43,11 @@ struct cipsi_ijkl_ctx {
     static constexpr int kOutSegments = 16;
     hipStream_t copy_stream = nullptr;
     hipEvent_t seg_ev[kOutSegments] = {};
-    hipEvent_t front_ev = nullptr;   // the last expand tile row run during the upload
+    // FP64 packed host entry point: the stream of the upload, the event of the last
+    // expand run behind it, and the event of the context stream's earlier work
+    hipStream_t up_stream = nullptr;
+    hipEvent_t front_ev = nullptr;
+    hipEvent_t entry_ev = nullptr;
     // Half-2 work tables, one per (N_MO, columns per work-group), written once and kept
     // until release_workspace: a graph captured from an earlier call keeps its pointer.
     struct TileTable { DevBuf buf; int64_t total = 0; };
