@@ -111,11 +111,11 @@ int cipsi_ijkl_autotune(cipsi_ijkl_ctx* ctx, int n_ao, int n_mo, int precision, 
     if ((rc = ctx_alloc(ctx, ctx->vsq, sizeof(double) * pa * pa))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->h, sizeof(double) * pa * pm))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->io_out, sizeof(double) * npairs(pm)))) return rc;
-    if ((rc = ctx_alloc(ctx, ctx->cr, sizeof(double) * (size_t)kCRows * mp))) return rc;
+    if ((rc = ctx_alloc(ctx, ctx->cr, sizeof(double) * (size_t)cr_pair_size(mp)))) return rc;
     double* vsq = (double*)ctx->vsq.ptr;
     double* h = (double*)ctx->h.ptr;
     if ((rc = fill(ctx, vsq, pa * pa, 1))) return rc;
-    if ((rc = fill(ctx, (double*)ctx->cr.ptr, (int64_t)kCRows * mp, 2))) return rc;
+    if ((rc = fill(ctx, (double*)ctx->cr.ptr, cr_pair_size(mp), 2))) return rc;
     const double* cr = (const double*)ctx->cr.ptr;
     int v1 = ctx->var_half1, v2 = ctx->var_half2;
     double t1 = DBL_MAX, t2 = DBL_MAX;

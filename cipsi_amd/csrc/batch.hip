@@ -301,7 +301,7 @@ int64_t cipsi_ijkl_batch_workspace_bytes(int n_geom, const int* n_ao, const int*
     for (int g = 0; g < n_geom; ++g) {
         const int64_t pa = npairs(n_ao[g]), pm = npairs(n_mo[g]);
         t += align256(8 * pa * pa) + align256(8 * pa * pm) +
-             align256(8 * (size_t)kCRows * round16(n_mo[g]));
+             align256(8 * (size_t)cr_pair_size(round16(n_mo[g])));
     }
     return t;
 }
@@ -357,7 +357,7 @@ int cipsi_ijkl_batch_create(cipsi_ijkl_ctx* ctx, int n_geom, const int* n_ao, co
         b->off_h.push_back(off);
         off += align256(8 * pa * pm);
         b->off_cr.push_back(off);
-        off += align256(8 * (size_t)kCRows * round16(b->m[g]));
+        off += align256(8 * (size_t)cr_pair_size(round16(b->m[g])));
     }
     b->work_bytes = off;
     // tables

@@ -133,7 +133,7 @@ int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half, int n, int m, int prec
 int run_pad_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_c, int ldc, double** cr) {
     const int np = round16(n), mp = round16(m);
     (void)np;
-    int rc = ctx_alloc(ctx, ctx->cr, sizeof(double) * (size_t)kCRows * mp);
+    int rc = ctx_alloc(ctx, ctx->cr, sizeof(double) * (size_t)cr_pair_size(mp));
     if (rc) return rc;
     StageTimer tm(ctx, "pad_c");
     HIP_TRY(ctx, launch_pad_c(d_c, ldc, n, m, kCRows, mp, (double*)ctx->cr.ptr, ctx->stream));
@@ -366,7 +366,7 @@ extern "C" {
 int64_t cipsi_ijkl_workspace_bytes(int n_ao, int n_mo, int precision, int with_labels) {
     if (check_dims(n_ao, n_mo, n_ao)) return -1;
     const int64_t pa = npairs(n_ao), pm = npairs(n_mo);
-    int64_t b = 8 * (pa * pa + pa * pm) + 8 * (int64_t)kCRows * round16(n_mo);
+    int64_t b = 8 * (pa * pa + pa * pm) + 8 * cr_pair_size(round16(n_mo));
     if (with_labels) b += 12 * npairs(pa);
     if (precision == CIPSI_PREC_SPLIT_FP16) b += split_workspace_bytes(n_ao, n_mo);
     return b;

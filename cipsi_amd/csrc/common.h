@@ -44,6 +44,17 @@ constexpr int kCRows = 256;
 __host__ __device__ inline int64_t cr_index(int r, int a) {
     return ((int64_t)(a >> 4) * kCRows + r) * 16 + (a & 15);
 }
+// Every buffer that holds the padded C holds a second copy right after it, kCRows * mp
+// doubles further on (pad_c writes both; cr_pair_size gives the doubles of both).  The
+// copy keeps the 16 x 16 blocks (row block rb, column block ab) but orders each one for
+// the B operand of the f64 MFMA in the second product, where lane l = 16 lg + lm needs
+// C(16 rb + 4 q + lg, 16 ab + lm) for q = 0..3: q = 2h + j sits at h * 128 + 2 l + j, so a
+// lane loads its four values as two 16-byte pieces and a wave's load is 1 KiB contiguous.
+__host__ __device__ inline int64_t cq_index(int r, int a) {
+    const int rr = r & 15, q = rr >> 2, l = 16 * (rr & 3) + (a & 15);
+    return ((int64_t)(a >> 4) * 16 + (r >> 4)) * 256 + (q >> 1) * 128 + 2 * l + (q & 1);
+}
+__host__ __device__ inline int64_t cr_pair_size(int mp) { return 2 * (int64_t)kCRows * mp; }
 
 struct HalfArgs {
     const double* src;     // rows of packed symmetric matrices

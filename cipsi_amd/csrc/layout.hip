@@ -4,7 +4,8 @@
 //  expand_packed  : packed lower-triangular AO block -> full symmetric pair matrix
 //  transpose      : H[pq][kl] -> H^T[kl][pq] between the two halves
 //  pad_c          : C (column-major, ldc) -> zero-padded copy in 16-column blocks of kCRows
-//                   rows (cr_index) read by the MFMA tiles
+//                   rows (cr_index) read by the MFMA tiles, and its second-product order
+//                   (cq_index)
 #include "common.h"
 
 namespace cipsi {
@@ -489,13 +490,16 @@ hipError_t launch_transpose(const double* src, int64_t rows, int64_t cols, doubl
 }
 
 // ----------------------------------------------------------------------------- C padding
-// np x mp padded copy of C in 16-column blocks (cr_index; rows >= n, columns >= m zero)
+// np x mp padded copy of C in 16-column blocks (cr_index; rows >= n, columns >= m zero),
+// and the same values in the second-product order (cq_index) kCRows * mp doubles further on
 __global__ void k_pad_c(const double* __restrict__ c, int ldc, int n, int m, int np, int mp,
                         double* __restrict__ cr) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)np * mp) return;
     int r = (int)(i / mp), a = (int)(i - (int64_t)r * mp);
-    cr[cr_index(r, a)] = (r < n && a < m) ? c[r + (int64_t)a * ldc] : 0.0;
+    const double v = (r < n && a < m) ? c[r + (int64_t)a * ldc] : 0.0;
+    cr[cr_index(r, a)] = v;
+    cr[(int64_t)kCRows * mp + cq_index(r, a)] = v;
 }
 
 hipError_t launch_pad_c(const double* c, int ldc, int n, int m, int np, int mp, double* cr,
@@ -550,7 +554,9 @@ __global__ __launch_bounds__(256) void k_group_pad_c(const LayoutGeom* __restric
     const LayoutGeom& G = g[k];
     const int64_t e = i - G.padc_start;
     const int r = (int)(e / G.mp), a = (int)(e - (int64_t)r * G.mp);
-    G.cr[cr_index(r, a)] = (r < G.n && a < G.m) ? G.c[r + (int64_t)a * G.ldc] : 0.0;
+    const double v = (r < G.n && a < G.m) ? G.c[r + (int64_t)a * G.ldc] : 0.0;
+    G.cr[cr_index(r, a)] = v;
+    G.cr[(int64_t)kCRows * G.mp + cq_index(r, a)] = v;
 }
 
 hipError_t launch_group_expand(const LayoutGeom* g, int ngeom, int64_t total_tiles, hipStream_t st) {

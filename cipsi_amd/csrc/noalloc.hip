@@ -23,14 +23,14 @@ namespace {
 constexpr int kNoAllocMaxAO = 48;
 constexpr int64_t kNaPairs = (int64_t)kNoAllocMaxAO * (kNoAllocMaxAO + 1) / 2;   // 1176
 constexpr int64_t kNaSq = kNaPairs * kNaPairs;           // doubles of the pair matrix / of H
-constexpr int64_t kNaCr = (int64_t)kCRows * kNoAllocMaxAO;   // padded C, N_MO <= N_AO
+constexpr int64_t kNaCr = 2 * (int64_t)kCRows * kNoAllocMaxAO;   // padded C and its second-product copy, N_MO <= N_AO
 constexpr int64_t kNaTab = 64;                           // half-2 work table (N_MO + 1 <= 49)
-// Layout in doubles: [pair matrix, later H^T | H | padded C | work table (int64)].
+// Layout in doubles: [pair matrix, later H^T | H | padded C, both orders | work table (int64)].
 // Every region starts on a 256-byte boundary (kNaSq and kNaCr are multiples of 32).
 constexpr int64_t kNaOffH = kNaSq;
 constexpr int64_t kNaOffCr = 2 * kNaSq;
 constexpr int64_t kNaOffTab = 2 * kNaSq + kNaCr;
-constexpr int64_t kNaDoubles = kNaOffTab + kNaTab;       // 2,778,304 doubles = 22.2 MB
+constexpr int64_t kNaDoubles = kNaOffTab + kNaTab;       // 2,790,592 doubles = 22.3 MB
 static_assert(kNaSq % 32 == 0 && kNaCr % 32 == 0, "regions must stay 256-byte aligned");
 
 constexpr int kMaxDevices = 64;

@@ -224,6 +224,7 @@ struct SymHalfDesc {
     int64_t src_ld;        // doubles per source row
     const double* crR;     // CR padded in 16-column blocks of kCRows rows (cr_index)
     const double* crL;     // CL, the same layout
+    const double* cqL;     // CL in the second-product order (cq_index)
     double* dst;
     int64_t dst_ld;        // half 1: doubles per output row
     int64_t nb;            // matrices
@@ -355,12 +356,14 @@ __device__ __forceinline__ void sym_tile(const SymHalfDesc& D, int64_t local) {
     double* __restrict__ dst = D.dst + rowoff;
     const int last_ab = D.tri_out ? cw / 16 : (D.ma - 1) / 16;
     for (int ab = 0; ab <= last_ab; ++ab) {
-        const double* __restrict__ cba = D.crL + (int64_t)ab * (kCRows * 16) + lg * 16 + lm;
+        // a lane's four C values of row block rb: two 16-byte pieces of the cq_index copy
+        typedef double f64x2 __attribute__((ext_vector_type(2)));
+        const f64x2* __restrict__ cqa = (const f64x2*)(D.cqL + (int64_t)ab * 16 * 256) + lane;
         f64x4 o0 = f64x4{0.0, 0.0, 0.0, 0.0}, o1 = o0;
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb) {
-            const double b0 = cba[(16 * rb + 0) * 16], b1 = cba[(16 * rb + 4) * 16];
-            const double b2 = cba[(16 * rb + 8) * 16], b3 = cba[(16 * rb + 12) * 16];
+            const f64x2 v0 = cqa[rb * 128], v1 = cqa[rb * 128 + 64];
+            const double b0 = v0.x, b1 = v0.y, b2 = v1.x, b3 = v1.y;
             o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][0], b0, o0, 0, 0, 0);
             o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][1], b1, o1, 0, 0, 0);
             o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][2], b2, o0, 0, 0, 0);
@@ -748,7 +751,7 @@ extern "C" int cipsi_ijkl_sym_transform_labels(
         c_off[s] = sym->ind[s] - 1;
         csz = std::max<int64_t>(csz, c_off[s] + (int64_t)nbp[s] * nbo[s]);
         cr_off[s] = crsz;
-        crsz += (int64_t)kCRows * ((nbo[s] + 15) & ~15) + 32;
+        crsz += cr_pair_size((nbo[s] + 15) & ~15) + 32;
     }
     if ((rc = ctx_alloc(ctx, ctx->io_c, sizeof(double) * std::max<int64_t>(csz, 1)))) return rc;
     if ((rc = ctx_alloc(ctx, ctx->cr, sizeof(double) * crsz))) return rc;
@@ -780,6 +783,7 @@ extern "C" int cipsi_ijkl_sym_transform_labels(
         SymHalfDesc& x = d1[b];
         x.src = d_vsq; x.src_ld = pa;
         x.crR = d_cr + cr_off[A]; x.crL = d_cr + cr_off[Bi];
+        x.cqL = x.crL + (int64_t)kCRows * ((nbo[Bi] + 15) & ~15);
         x.dst = d_h + B.h_off; x.dst_ld = B.nlk;
         x.nb = B.npq;
         x.nr = nbp[Bi]; x.ns = nbp[A]; x.mc = nbo[A]; x.ma = nbo[Bi];
@@ -792,6 +796,7 @@ extern "C" int cipsi_ijkl_sym_transform_labels(
         SymHalfDesc& y = d2[b];
         y.src = d_ht + B.h_off; y.src_ld = B.npq;
         y.crR = d_cr + cr_off[C]; y.crL = d_cr + cr_off[D];
+        y.cqL = y.crL + (int64_t)kCRows * ((nbo[D] + 15) & ~15);
         y.dst = d_out + B.out_off; y.dst_ld = 0;
         y.nb = B.nlk;
         y.nr = nbp[D]; y.ns = nbp[C]; y.mc = nbo[C]; y.ma = nbo[D];

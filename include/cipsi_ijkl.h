@@ -111,7 +111,7 @@ int cipsi_ijkl_transform_labels_dev(cipsi_ijkl_ctx *ctx, int n_ao, int n_mo, con
 /* The same FP64 transform as cipsi_ijkl_transform_packed_dev (same kernels, tile plan and
    bits) for callers that cannot allocate device memory: code under graph capture, tight
    serving loops, harnesses that forbid hipMalloc.  No counterpart in the reference.  It
-   calls no hipMalloc / hipFree: its workspace is a static 22.2 MB device array, one copy
+   calls no hipMalloc / hipFree: its workspace is a static 22.3 MB device array, one copy
    per GPU, held until the process ends, which limits it to n_ao <= 48
    (CIPSI_ERR_UNSUPPORTED above).  FP64 only.  Asynchronous on the context stream.  Every
    call on one GPU uses that GPU's single copy, so calls that overlap on one GPU (two

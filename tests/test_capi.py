@@ -34,7 +34,8 @@ def test_strerror_and_workspace_without_gpu():
     assert b"label" in lib.cipsi_ijkl_strerror(-2)
     # 240 / 240 FP64: symmetric pair matrix + H + padded C
     pa = 240 * 241 // 2
-    assert ijkl.workspace_bytes(240, 240) == 8 * (pa * pa + pa * pa) + 8 * 256 * 240   # C padded to 256 rows
+    # C padded to 256 rows, in both orders (first and second product)
+    assert ijkl.workspace_bytes(240, 240) == 8 * (pa * pa + pa * pa) + 2 * 8 * 256 * 240
     assert ijkl.workspace_bytes(10, 11) == -1          # n_mo > n_ao rejected
     assert ijkl.workspace_bytes(300, 10) == -1         # beyond the supported N_AO
 
