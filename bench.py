@@ -47,11 +47,11 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload
 # (fetched = 2.0 x FETCH_SIZE, written = 1.0 x WRITE_SIZE, factors from a 1 GiB copy in
 # the same process); keyed by (n_ao, n_mo, precision).  FP64: one launch of half 1 (H^T
-# stores), KiB averaged over four launches.  Split: the split kernels of both halves of one
+# stores), KiB averaged over three launches.  Split: the split kernels of both halves of one
 # transform (row exponents, step 1, step 2).  Summaries: profiles/fp64_n240_kernels.md,
 # profiles/split_fp16_n240_kernels.md.
-MEASURED_TRAFFIC = {(240, 240, "fp64"): (2.0 * 4058352.0 + 14190385.0) * 1024,
-                    (240, 240, "split_fp16"): (2.0 * 34947414.0 + 34269310.0) * 1024}
+MEASURED_TRAFFIC = {(240, 240, "fp64"): (2.0 * 5189059.0 + 11583414.0) * 1024,
+                    (240, 240, "split_fp16"): (2.0 * 35084084.0 + 33311744.0) * 1024}
 # k_gather over 2^27 uniformly random labels at N_MO = 240, per launch: 2.0 x FETCH_SIZE
 # 8.6412e6 KiB + 1.0 x WRITE_SIZE 1.0486e6 KiB (profiles/consumers_n240.md).
 MEASURED_TRAFFIC_GATHER = {(240, 1 << 27): (2.0 * 8.6412e6 + 1.0486e6) * 1024}

@@ -18,7 +18,9 @@
 // i + j = d are accumulated together, group d = 2..S+1 (S(S+1)/2 digit products in
 // all; the groups of five and six products have a relative weight 2^-32 or less and
 // their FP32 rounding is negligible).  The epilogue forms
-//   out = sum_d 2^(e_row + f_col + 2 - 8d) acc_d  in FP64.
+//   out = sum_d 2^(e_row + f_col + 2 - 8d) acc_d  in FP64,
+// by Horner's rule in 2^-8 (one FMA per group and one ldexp, instead of an ldexp and an
+// add per group).
 // The slice count S (3..6) sets the accuracy: the truncation error of an operand is
 // 2^-8S of its row maximum.
 //
@@ -412,10 +414,11 @@ __global__ __launch_bounds__(NW2 * 64, 2) void k_split_tri(SplitArgs A) {
             for (int q = 0; q < 4; ++q) {
                 const int il = 16 * rb + 4 * lg + q;
                 const int base = rx[il] + fc + 2;
-                double v = 0.0;
+                double v = (double)acc[S - 1][rb][cb][q];
 #pragma unroll
-                for (int g = S - 1; g >= 0; --g)
-                    v += ldexp((double)acc[g][rb][cb][q], base - 8 * (g + 2));
+                for (int g = S - 2; g >= 0; --g)
+                    v = fma(v, 0.00390625, (double)acc[g][rb][cb][q]);
+                v = ldexp(v, base - 16);
                 const int r = row0 + il;
                 if (r < nrows && col <= r) {
                     const int64_t p = pair0(r, col);
@@ -590,10 +593,11 @@ __global__ __launch_bounds__(NW1 * 64, 1) void k_split_rows(SplitArgs A) {
                         for (int q = 0; q < 4; ++q) {
                             const int il = 16 * rb + 4 * lg + q;
                             const int base = er[rb][q] + fc[cb] + 2;
-                            double v = 0.0;
+                            double v = (double)acc[S - 1][rb][cb][q];
 #pragma unroll
-                            for (int gg = S - 1; gg >= 0; --gg)
-                                v += ldexp((double)acc[gg][rb][cb][q], base - 8 * (gg + 2));
+                            for (int gg = S - 2; gg >= 0; --gg)
+                                v = fma(v, 0.00390625, (double)acc[gg][rb][cb][q]);
+                            v = ldexp(v, base - 16);
                             const int r = row0 + il;
                             if (r < A.n && col < ncols) {
                                 dst[(b * A.mp + col) * (int64_t)A.kp + r] = v;
