@@ -123,6 +123,40 @@ int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int6
     return ctx_prefix_table(ctx, std::make_pair(m, lt), m, column_tiles, out, total);
 }
 
+// Work items of the paired FP64 half 2 (build_pair_items) for N_MO = m, kept like the
+// prefix tables (key (m, -2)): built and uploaded on first use, never rewritten.
+static int ctx_pair_table(cipsi_ijkl_ctx* ctx, int m, const PairItem** out, int64_t* total) {
+    const auto key = std::make_pair(m, -2);
+    auto it = ctx->tile_tables.find(key);
+    if (it == ctx->tile_tables.end()) {
+        std::vector<PairItem> items;
+        try {
+            build_pair_items(m, items);
+        } catch (...) {
+            return CIPSI_ERR_NOMEM;
+        }
+        cipsi_ijkl_ctx::TileTable tab;
+        int rc = ctx_alloc(ctx, tab.buf, sizeof(PairItem) * std::max<size_t>(items.size(), 1));
+        if (rc) return rc;
+        hipError_t e = hipMemcpy(tab.buf.ptr, items.data(), sizeof(PairItem) * items.size(),
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(tab.buf.ptr);
+            return map_hip(ctx, e, "hipMemcpy(pair table)");
+        }
+        tab.total = (int64_t)items.size();
+        try {
+            it = ctx->tile_tables.emplace(key, tab).first;
+        } catch (...) {
+            (void)hipFree(tab.buf.ptr);
+            return CIPSI_ERR_NOMEM;
+        }
+    }
+    *out = (const PairItem*)it->second.buf.ptr;
+    *total = it->second.total;
+    return CIPSI_OK;
+}
+
 int ctx_half_variant(const cipsi_ijkl_ctx* ctx, int half, int n, int m, int precision) {
     auto it = ctx->plans.find(std::make_tuple(n, m, precision));
     if (it != ctx->plans.end()) return half == 1 ? it->second.first : it->second.second;
@@ -183,7 +217,18 @@ int run_half_variant(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* 
         a.tile_start = ts;
         a.ntile_total = total;
     }
+    const PairItem* pitems = nullptr;
+    int64_t npitems = 0;
+    const bool paired = precision == CIPSI_PREC_FP64 && half == 2 && var == kPairedHalf2Variant;
+    if (paired) {
+        int rc = ctx_pair_table(ctx, m, &pitems, &npitems);
+        if (rc) return rc;
+    }
     StageTimer tm(ctx, half == 1 ? "half1" : "half2");
+    if (paired) {
+        HIP_TRY(ctx, launch_half2_pair(a, pitems, npitems, ctx->stream));
+        return CIPSI_OK;
+    }
     if (precision == CIPSI_PREC_FP64) {
         HIP_TRY(ctx, launch_half_fp64(a, var, ctx->stream));
         return CIPSI_OK;

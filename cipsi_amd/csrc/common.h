@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace cipsi {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -110,6 +112,19 @@ hipError_t launch_transpose(const double* src, int64_t rows, int64_t cols, doubl
 hipError_t launch_pad_c(const double* c, int ldc, int n, int m, int np, int mp, double* cr,
                         hipStream_t st);
 hipError_t launch_half_fp64(const HalfArgs& a, int variant, hipStream_t st);
+
+// Half 2 of one transform with the column blocks of its matrices regrouped into fuller work
+// groups (half_fp64.hip): an item holds one or two pieces of matrices; slots byte w says what
+// wave w does (bit 7 busy, bit 4 the piece, bits 0-3 the column block).
+struct PairItem {
+    int64_t b[2];          // matrices (MO pairs) of the pieces; b[1] < 0: one piece
+    uint64_t slots;
+};
+hipError_t launch_half2_pair(const HalfArgs& a, const PairItem* items, int64_t nitems,
+                             hipStream_t st);
+void build_pair_items(int m, std::vector<PairItem>& out);
+// The tile variant (8 waves, panel depth 16) whose single-transform half 2 runs paired.
+constexpr int kPairedHalf2Variant = 2;
 
 // Row class (compile-time row-block count) used for a padded AO dimension np.
 int row_class(int np);

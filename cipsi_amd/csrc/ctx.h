@@ -48,7 +48,8 @@ struct cipsi_ijkl_ctx {
     hipStream_t up_stream = nullptr;
     hipEvent_t front_ev = nullptr;
     hipEvent_t entry_ev = nullptr;
-    // Half-2 work tables, one per (N_MO, columns per work-group), written once and kept
+    // Half-2 work tables, one per (N_MO, kind): kind > 0 the FP64 columns per work-group,
+    // -1 the split path's triangle items, -2 the paired FP64 items.  Written once and kept
     // until release_workspace: a graph captured from an earlier call keeps its pointer.
     struct TileTable { DevBuf buf; int64_t total = 0; };
     std::map<std::pair<int, int>, TileTable> tile_tables;
@@ -107,7 +108,7 @@ int ctx_copy_h2d(cipsi_ijkl_ctx* ctx, void* d, const void* h, size_t bytes);
 int ctx_enter(cipsi_ijkl_ctx* ctx);
 int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int64_t* total);
 // A half-2 prefix table keyed by (m, kind): kind > 0 is the FP64 column width; the split
-// path uses kind < 0.  items(l, kind) is the number of work items of one MO pair of row l.
+// path uses kind -1.  items(l, kind) is the number of work items of one MO pair of row l.
 int ctx_prefix_table(cipsi_ijkl_ctx* ctx, std::pair<int, int> key, int m, int64_t (*items)(int, int),
                      const int64_t** out, int64_t* total);
 int run_pad_c(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_c, int ldc, double** cr);

@@ -24,6 +24,9 @@
 // from cr_index order, the second from cq_index order.
 #include "common.h"
 
+#include <utility>
+#include <vector>
+
 namespace cipsi {
 
 template <int NW, int NRB, int KC>
@@ -283,6 +286,206 @@ int64_t xcd_grid(int64_t items) {
     return (items + q - 1) / q * q;
 }
 
+// ---------------------------------------------------------------- half 2, paired pieces
+// Half 2 of one transform with the column blocks of its matrices regrouped (8 waves, panel
+// depth 16).  An MO pair of row l has nb = ceil((l + 1) / 16) column blocks.  A matrix with
+// more than 8 blocks gives one full piece of 8 blocks and a remainder of nb - 8; a matrix with
+// fewer than 8 gives one partial piece.  Two pieces whose blocks add up to at most 8 share a
+// work-group: each is staged into its own pair of LDS panel buffers, and every wave works on
+// one block of one piece, with Q1 and Q2 exactly as in half_tile (the same operations in the
+// same order, so the results are the same bits).  A work-group's time depends little on how
+// many of its waves are busy, so fewer, fuller work-groups finish half 2 sooner.
+// The remainder of a large matrix takes the middle blocks [4, 4 + r): the full piece keeps
+// blocks 0..3 and nb - 4..nb - 1, whose Q2 triangles pair up evenly on the SIMDs.
+template <int NRB>
+__device__ __forceinline__ void pair_tile(const HalfArgs& A, const PairItem& it) {
+    constexpr int NW = 8, KC = 16;
+    using K = HalfCfg<NW, NRB, KC>;
+    extern __shared__ double smem[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int lm = lane & 15, lg = lane >> 4;
+    const int np = A.np, n = A.n;
+    constexpr int RP = NRB * 16;
+    constexpr int LDZ = RP + 1;
+    constexpr int PANEL = KC * LDZ;
+    const bool two = it.b[1] >= 0;
+    const unsigned slot = (unsigned)(it.slots >> (8 * w)) & 0xFFu;
+    const bool wave_active = (slot & 0x80u) != 0;
+    const int piece = (int)((slot >> 4) & 1u);
+    const int cw = 16 * (int)(slot & 15u);             // first column of this wave
+    const int64_t b = it.b[piece];                     // the wave's matrix (an MO pair)
+    const int64_t l = wave_active ? pair_row(b) : 0;
+    const int c1 = (int)l + 1;                         // columns j <= l
+    const int64_t limit = b + 1, dst_off = pair0(b, 0);
+    gcdouble* __restrict__ cr = (gcdouble*)A.cr;
+    gcdouble* __restrict__ zsrc0 = (gcdouble*)A.src + it.b[0] * A.src_ld;
+    gcdouble* __restrict__ zsrc1 = (gcdouble*)A.src + (two ? it.b[1] : it.b[0]) * A.src_ld;
+
+    double zst0[K::NU], zst1[K::NU];
+    auto zslot = [&](int u, int& p, int& e) {
+        const int sweep = u / K::SUB, sub = u % K::SUB;
+        const int t = tid + sub * K::NT;
+        p = sweep * K::PPS + t / K::EP;
+        e = t % K::EP;
+    };
+    auto zmap = [&](int p, int e, int s0, int& r, int& kk) {
+        const bool above = 16 * p + 15 < s0;
+        r = 16 * p + (above ? (e & 15) : e / KC);
+        kk = above ? (e >> 4) : (e % KC);
+    };
+    auto fetch = [&](int s0, gcdouble* __restrict__ zsrc, double (&zst)[K::NU]) {
+#pragma unroll
+        for (int u = 0; u < K::NU; ++u) {
+            int p, e, r, kk;
+            zslot(u, p, e);
+            zmap(p, e, s0, r, kk);
+            const int s = s0 + kk;
+            const unsigned hi = r >= s ? r : s, lo = r >= s ? s : r;
+            const unsigned idx = ((hi * (hi + 1)) >> 1) + lo;
+            gcdouble* ptr = (r < n && s < n) ? zsrc + idx : (gcdouble*)kZeroElement;
+            zst[u] = *ptr;
+        }
+    };
+    auto commit = [&](int s0, int boff, const double (&zst)[K::NU]) {
+#pragma unroll
+        for (int u = 0; u < K::NU; ++u) {
+            int p, e, r, kk;
+            zslot(u, p, e);
+            zmap(p, e, s0, r, kk);
+            if (p < NRB) smem[boff + kk * LDZ + r] = zst[u];
+        }
+    };
+
+    f64x4 acc[NRB];
+#pragma unroll
+    for (int i = 0; i < NRB; ++i) acc[i] = f64x4{0.0, 0.0, 0.0, 0.0};
+    constexpr int NB = KC / 4;
+    gcdouble* __restrict__ cbw = cr + (int64_t)(cw >> 4) * (kCRows * 16) + lg * 16 + lm;
+    double bcur[NB];
+    const int nchunk = (np + KC - 1) / KC;
+    if (wave_active) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) bcur[k] = cbw[(4 * k) * 16];
+    }
+    // panel buffers: piece p, chunk parity q at (2 p + q) * PANEL
+    fetch(0, zsrc0, zst0);
+    if (two) fetch(0, zsrc1, zst1);
+    commit(0, 0, zst0);
+    if (two) commit(0, 2 * PANEL, zst1);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __syncthreads();
+    for (int ch = 0; ch < nchunk; ++ch) {
+        const bool more = (ch + 1) < nchunk;
+        double bnext[NB];
+        if (more && wave_active) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) bnext[k] = cbw[((ch + 1) * KC + 4 * k) * 16];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (more) {
+            fetch((ch + 1) * KC, zsrc0, zst0);
+            if (two) fetch((ch + 1) * KC, zsrc1, zst1);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (wave_active) {
+            const int zo = (2 * piece + (ch & 1)) * PANEL;
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+#pragma unroll
+                for (int rb = 0; rb < NRB; ++rb) {
+                    const double af = smem[zo + (4 * k + lg) * LDZ + rb * 16 + lm];
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bcur[k], acc[rb], 0, 0, 0);
+                }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (more) {
+            commit((ch + 1) * KC, ((ch + 1) & 1) * PANEL, zst0);
+            if (two) commit((ch + 1) * KC, (2 + ((ch + 1) & 1)) * PANEL, zst1);
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        if (more && wave_active) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) bcur[k] = bnext[k];
+        }
+    }
+    if (!wave_active) return;
+
+    // Q4 as in half_tile: O^T = T^T C, a <= c, pair0(c, a) < limit
+    const int my_last_ab = cw / 16;
+    gdouble* __restrict__ dst = (gdouble*)A.dst + dst_off;
+    constexpr int QD = 3;
+    double ring[QD + 1][4];
+    gcdouble* __restrict__ cq = cr + (int64_t)kCRows * A.ldm;
+    auto load_rb = [&](int ab, int rb, double* bq) {
+        gcdouble2* __restrict__ p = (gcdouble2*)(cq + ((int64_t)ab * 16 + rb) * 256) + lane;
+        const f64x2 v0 = p[0], v1 = p[64];
+        bq[0] = v0.x;
+        bq[1] = v0.y;
+        bq[2] = v1.x;
+        bq[3] = v1.y;
+    };
+#pragma unroll
+    for (int rb = 0; rb < QD && rb < NRB; ++rb) load_rb(0, rb, ring[rb]);
+    for (int ab = 0; ab <= my_last_ab; ++ab) {
+        f64x4 o0 = f64x4{0.0, 0.0, 0.0, 0.0}, o1 = o0;
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb) {
+            if (rb + QD < NRB) load_rb(ab, rb + QD, ring[(rb + QD) % (QD + 1)]);
+            const double* bq = ring[rb % (QD + 1)];
+            o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][0], bq[0], o0, 0, 0, 0);
+            o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][1], bq[1], o1, 0, 0, 0);
+            o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][2], bq[2], o0, 0, 0, 0);
+            o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[rb][3], bq[3], o1, 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (ab < my_last_ab) {
+#pragma unroll
+            for (int rb = 0; rb < QD && rb < NRB; ++rb) load_rb(ab + 1, rb, ring[rb]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const int a = ab * 16 + lm;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = cw + lg + 4 * q;
+            if (c < c1 && a <= c) {
+                const int64_t pp = pair0(c, a);
+                if (pp < limit) dst[pp] = o0[q] + o1[q];
+            }
+        }
+    }
+}
+
+template <int NRB>
+__global__ __launch_bounds__(512, 1) void k_half2_pair(HalfArgs A, const PairItem* __restrict__ items,
+                                                      int64_t nitems) {
+    const int64_t item = xcd_item(blockIdx.x);
+    if (item >= nitems) return;
+    const PairItem it = items[item];
+    pair_tile<NRB>(A, it);
+}
+
+template <int NRB>
+static hipError_t launch_pair_rows(const HalfArgs& a, const PairItem* items, int64_t nitems,
+                                   hipStream_t st) {
+    const size_t shmem = sizeof(double) * 4 * 16 * (NRB * 16 + 1);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_half2_pair<NRB>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const int64_t g = xcd_grid(nitems);
+    if (g == 0) return hipSuccess;
+    if (g > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_half2_pair<NRB>), dim3((unsigned)g), dim3(512), shmem, st, a, items, nitems);
+    return hipGetLastError();
+}
+
+
 template <int NW, int NRB, int KC, int MINB>
 __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
     const int64_t item = A.item_begin + xcd_item(blockIdx.x);
@@ -418,6 +621,109 @@ hipError_t launch_half_fp64(const HalfArgs& a, int variant, hipStream_t st) {
 #undef CIPSI_CASE
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_half2_pair(const HalfArgs& a, const PairItem* items, int64_t nitems,
+                             hipStream_t st) {
+    if (a.half != 2 || a.np > 256 || (a.np & 15) || (a.mp & 15)) return hipErrorInvalidValue;
+    switch (row_class(a.np)) {
+#define CIPSI_CASE(R) case R: return launch_pair_rows<R>(a, items, nitems, st);
+        CIPSI_ROW_CLASSES(CIPSI_CASE)
+#undef CIPSI_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Work items of the paired half 2 (see pair_tile), in the order of the MO pair rows: full
+// pieces as their matrix comes, a partial piece or remainder when a waiting piece completes
+// it to 8 blocks, and the pieces still waiting at the end alone.  Within an item the waves
+// take the blocks as in half_tile: sorted by block, SIMD j < P = nact - 4 holds the j-th
+// smallest and the j-th largest, the other SIMDs one block each.
+void build_pair_items(int m, std::vector<PairItem>& out) {
+    out.clear();
+    struct Piece { int64_t b; int first, count; };   // blocks [first, first + count) of b
+    std::vector<Piece> waiting[8];
+    auto emit = [&](const Piece* p, int np_) {
+        // entries (block, piece), sorted by block
+        int blk[8], pc[8], ne = 0;
+        for (int i = 0; i < np_; ++i)
+            for (int k = 0; k < p[i].count; ++k) { blk[ne] = p[i].first + k; pc[ne] = i; ++ne; }
+        for (int i = 1; i < ne; ++i)
+            for (int j = i; j > 0 && blk[j] < blk[j - 1]; --j) {
+                std::swap(blk[j], blk[j - 1]);
+                std::swap(pc[j], pc[j - 1]);
+            }
+        PairItem it;
+        it.b[0] = p[0].b;
+        it.b[1] = np_ > 1 ? p[1].b : -1;
+        it.slots = 0;
+        const int p2 = ne > 4 ? ne - 4 : 0;
+        for (int wv = 0; wv < 8; ++wv) {
+            const int j = wv & 3;
+            int e;
+            if (j < p2) e = (wv >> 2) ? 2 * p2 - 1 - j : j;
+            else e = (wv >> 2) ? -1 : p2 + j;
+            if (e < 0 || e >= ne) continue;
+            const uint64_t code = 0x80u | ((unsigned)pc[e] << 4) | (unsigned)blk[e];
+            it.slots |= code << (8 * wv);
+        }
+        out.push_back(it);
+    };
+    auto add_partial = [&](Piece pc) {
+        const int s = pc.count;
+        if (s == 8) {
+            emit(&pc, 1);
+        } else if (!waiting[8 - s].empty()) {
+            Piece two[2] = {waiting[8 - s].back(), pc};
+            waiting[8 - s].pop_back();
+            emit(two, 2);
+        } else {
+            waiting[s].push_back(pc);
+        }
+    };
+    for (int l = 0; l < m; ++l) {
+        const int nb = (l + 16) / 16;
+        for (int k = 0; k <= l; ++k) {
+            const int64_t b = pair0(l, k);
+            if (nb == 8) {
+                Piece p{b, 0, 8};
+                emit(&p, 1);
+            } else if (nb < 8) {
+                add_partial(Piece{b, 0, nb});
+            } else {
+                // full piece: blocks 0..3 and 4 + r .. nb - 1; remainder: blocks 4 .. 3 + r
+                const int r = nb - 8;
+                PairItem it;
+                it.b[0] = b;
+                it.b[1] = -1;
+                it.slots = 0;
+                int blk[8];
+                for (int i = 0; i < 4; ++i) blk[i] = i;
+                for (int i = 0; i < 4; ++i) blk[4 + i] = 4 + r + i;
+                for (int j = 0; j < 4; ++j) {           // SIMD j: the j-th smallest and largest
+                    it.slots |= (uint64_t)(0x80u | (unsigned)blk[j]) << (8 * j);
+                    it.slots |= (uint64_t)(0x80u | (unsigned)blk[7 - j]) << (8 * (j + 4));
+                }
+                out.push_back(it);
+                add_partial(Piece{b, 4, r});
+            }
+        }
+    }
+    // leftovers: pair any two that fit, largest first, then singles
+    for (int s = 7; s >= 1; --s)
+        while (!waiting[s].empty()) {
+            Piece a0 = waiting[s].back();
+            waiting[s].pop_back();
+            int t = 8 - s;
+            while (t >= 1 && waiting[t].empty()) --t;
+            if (t >= 1) {
+                Piece two[2] = {a0, waiting[t].back()};
+                waiting[t].pop_back();
+                emit(two, 2);
+            } else {
+                emit(&a0, 1);
+            }
+        }
 }
 
 hipError_t launch_half_fp64_group(const GroupLaunch& gl, hipStream_t st) {

@@ -174,6 +174,32 @@ def test_every_tile_variant_gives_the_same_bits(ctx, n, m):
         ctx.set_plan(n, m, *ctx.get_plan(1, 1))    # back to the context defaults
 
 
+@pytest.mark.parametrize("n,m", [(8, 5), (60, 33), (144, 129), (250, 240), (256, 256)])
+def test_paired_half2_gives_the_same_bits(ctx, n, m):
+    """The default 8-wave tile runs a single transform's half 2 with the column blocks
+    regrouped: a matrix of more than 8 blocks gives a full piece of 8 and a remainder, and
+    two pieces of at most 8 blocks together share a work-group.  The sizes give rows of
+    1 to 16 column blocks (remainders of every size, N_MO = 256 a remainder of 8).  The
+    output equals the unpaired half 2 of the 8-wave, panel-depth-32 tile bit for bit, and
+    the smallest size equals the oracle."""
+    print(f"[paired half 2] n={n} m={m}", flush=True)
+    case = synth.make_case(n, m, 2 * n, 8.0, 900 + n + m)
+    V, c = dev(case["ao_packed"]), dev(case["c"])
+    ref = ctx.transform(V, c)
+    torch.cuda.synchronize()
+    if n <= 8:
+        assert np.max(np.abs(ref.cpu().numpy() - oracle.ijkl(case["ao_packed"], case["c"]))) <= TOL
+    try:
+        ctx.set_plan(n, m, 2, 3)                 # half 2 on the unpaired panel-depth-32 tile
+        out = ctx.transform(V, c)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref)
+    finally:
+        ctx.set_plan(n, m, *ctx.get_plan(1, 1))
+    del V, out, ref
+    torch.cuda.empty_cache()
+
+
 def test_label_stream_duplicates_and_stale_tail(ctx):
     """transd semantics on the device, bit for bit against the oracle (transd.f:85-97,
     369-372): later duplicates overwrite earlier ones; the first zero label ends the
