@@ -124,14 +124,22 @@ int ctx_tile_table(cipsi_ijkl_ctx* ctx, int m, int lt, const int64_t** out, int6
 }
 
 // Work items of the paired FP64 half 2 (build_pair_items) for N_MO = m, kept like the
-// prefix tables (key (m, -2)): built and uploaded on first use, never rewritten.
-static int ctx_pair_table(cipsi_ijkl_ctx* ctx, int m, const PairItem** out, int64_t* total) {
-    const auto key = std::make_pair(m, -2);
+// prefix tables: built and uploaded on first use, never rewritten.  kind -2 holds the whole
+// half; kind -3 the row ranges [bound[s], bound[s + 1]) of the host entry point, items of
+// range s from seg[s] (bound has nseg + 1 entries).
+static int ctx_pair_table(cipsi_ijkl_ctx* ctx, int m, int kind, const int* bound, int nseg,
+                          const PairItem** out, const cipsi_ijkl_ctx::TileTable** tab_out) {
+    const auto key = std::make_pair(m, kind);
     auto it = ctx->tile_tables.find(key);
     if (it == ctx->tile_tables.end()) {
         std::vector<PairItem> items;
+        std::vector<int64_t> seg;
         try {
-            build_pair_items(m, items);
+            for (int r = 0; r < nseg; ++r) {
+                seg.push_back((int64_t)items.size());
+                build_pair_items(bound[r], bound[r + 1], items);
+            }
+            seg.push_back((int64_t)items.size());
         } catch (...) {
             return CIPSI_ERR_NOMEM;
         }
@@ -146,14 +154,15 @@ static int ctx_pair_table(cipsi_ijkl_ctx* ctx, int m, const PairItem** out, int6
         }
         tab.total = (int64_t)items.size();
         try {
-            it = ctx->tile_tables.emplace(key, tab).first;
+            tab.seg = std::move(seg);
+            it = ctx->tile_tables.emplace(key, std::move(tab)).first;
         } catch (...) {
             (void)hipFree(tab.buf.ptr);
             return CIPSI_ERR_NOMEM;
         }
     }
     *out = (const PairItem*)it->second.buf.ptr;
-    *total = it->second.total;
+    *tab_out = &it->second;
     return CIPSI_OK;
 }
 
@@ -221,8 +230,11 @@ int run_half_variant(cipsi_ijkl_ctx* ctx, int half, int n, int m, const double* 
     int64_t npitems = 0;
     const bool paired = precision == CIPSI_PREC_FP64 && half == 2 && var == kPairedHalf2Variant;
     if (paired) {
-        int rc = ctx_pair_table(ctx, m, &pitems, &npitems);
+        const int whole[2] = {0, m};
+        const cipsi_ijkl_ctx::TileTable* tab;
+        int rc = ctx_pair_table(ctx, m, -2, whole, 1, &pitems, &tab);
         if (rc) return rc;
+        npitems = tab->total;
     }
     StageTimer tm(ctx, half == 1 ? "half1" : "half2");
     if (paired) {
@@ -670,7 +682,7 @@ static int copy_d2h(cipsi_ijkl_ctx* ctx, void* h, const void* d, size_t bytes,
 // FP64 half 2 of a host entry point with the copy of its output to the caller running
 // behind it.  The MO stream is ordered by MO pair row kl = pair0(l, k), so the rows with
 // l < L are the prefix of npairs(npairs(L)) elements, and half 2's work items are ordered
-// by l too.  Half 2 is launched in up to kOutSegments ranges of l, each holding an equal
+// by l too (with the default tile, the paired items of each range of rows).  Half 2 is launched in up to kOutSegments ranges of l, each holding an equal
 // share of the output and each followed by an event; a second stream copies the stream
 // chunk by chunk through the pinned slots, every chunk after the event of the segment
 // that completes it.  The PCIe copy (about 60 ms at N_MO = 240) then starts a few
@@ -720,12 +732,22 @@ static int half2_stream_out(cipsi_ijkl_ctx* ctx, int n, int m, const double* ht,
     a.half = 2;
     a.nb = pm;
     a.tile_start = ts;
+    // the default tile runs each range of rows with its column blocks paired (pair_tile)
+    const PairItem* pitems = nullptr;
+    const cipsi_ijkl_ctx::TileTable* ptab = nullptr;
+    if (var == kPairedHalf2Variant && (rc = ctx_pair_table(ctx, m, -3, bound, nseg, &pitems, &ptab)))
+        return rc;
     {
         StageTimer tm(ctx, "half2");
         for (int s = 0; s < nseg; ++s) {
-            a.item_begin = items[bound[s]];
-            a.ntile_total = items[bound[s + 1]];
-            HIP_TRY(ctx, launch_half_fp64(a, var, ctx->stream));
+            if (ptab) {
+                HIP_TRY(ctx, launch_half2_pair(a, pitems + ptab->seg[s], ptab->seg[s + 1] - ptab->seg[s],
+                                               ctx->stream));
+            } else {
+                a.item_begin = items[bound[s]];
+                a.ntile_total = items[bound[s + 1]];
+                HIP_TRY(ctx, launch_half_fp64(a, var, ctx->stream));
+            }
             HIP_TRY(ctx, hipEventRecord(ctx->seg_ev[s], ctx->stream));
         }
     }

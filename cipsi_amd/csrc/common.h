@@ -122,7 +122,7 @@ struct PairItem {
 };
 hipError_t launch_half2_pair(const HalfArgs& a, const PairItem* items, int64_t nitems,
                              hipStream_t st);
-void build_pair_items(int m, std::vector<PairItem>& out);
+void build_pair_items(int l0, int l1, std::vector<PairItem>& out);
 // The tile variant (8 waves, panel depth 16) whose single-transform half 2 runs paired.
 constexpr int kPairedHalf2Variant = 2;
 

@@ -49,9 +49,10 @@ struct cipsi_ijkl_ctx {
     hipEvent_t front_ev = nullptr;
     hipEvent_t entry_ev = nullptr;
     // Half-2 work tables, one per (N_MO, kind): kind > 0 the FP64 columns per work-group,
-    // -1 the split path's triangle items, -2 the paired FP64 items.  Written once and kept
+    // -1 the split path's triangle items, -2 the paired FP64 items of the whole half, -3
+    // those of the host entry point's row ranges (seg: first item of each range).  Written once and kept
     // until release_workspace: a graph captured from an earlier call keeps its pointer.
-    struct TileTable { DevBuf buf; int64_t total = 0; };
+    struct TileTable { DevBuf buf; int64_t total = 0; std::vector<int64_t> seg; };
     std::map<std::pair<int, int>, TileTable> tile_tables;
     // Buffers replaced by a larger allocation.  They are not freed before
     // release_workspace / destroy, so a captured graph that still names them stays valid.

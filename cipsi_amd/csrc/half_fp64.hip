@@ -634,13 +634,13 @@ hipError_t launch_half2_pair(const HalfArgs& a, const PairItem* items, int64_t n
     }
 }
 
-// Work items of the paired half 2 (see pair_tile), in the order of the MO pair rows: full
-// pieces as their matrix comes, a partial piece or remainder when a waiting piece completes
-// it to 8 blocks, and the pieces still waiting at the end alone.  Within an item the waves
-// take the blocks as in half_tile: sorted by block, SIMD j < P = nact - 4 holds the j-th
-// smallest and the j-th largest, the other SIMDs one block each.
-void build_pair_items(int m, std::vector<PairItem>& out) {
-    out.clear();
+// Work items of the paired half 2 (see pair_tile) for the MO pair rows l0 <= l < l1,
+// appended to out in the order of the rows: full pieces as their matrix comes, a partial
+// piece or remainder when a waiting piece completes it to 8 blocks, and the pieces still
+// waiting at the end paired largest first or alone.  Within an item the waves take the
+// blocks as in half_tile: sorted by block, SIMD j < P = nact - 4 holds the j-th smallest
+// and the j-th largest, the other SIMDs one block each.
+void build_pair_items(int l0, int l1, std::vector<PairItem>& out) {
     struct Piece { int64_t b; int first, count; };   // blocks [first, first + count) of b
     std::vector<Piece> waiting[8];
     auto emit = [&](const Piece* p, int np_) {
@@ -681,7 +681,7 @@ void build_pair_items(int m, std::vector<PairItem>& out) {
             waiting[s].push_back(pc);
         }
     };
-    for (int l = 0; l < m; ++l) {
+    for (int l = l0; l < l1; ++l) {
         const int nb = (l + 16) / 16;
         for (int k = 0; k <= l; ++k) {
             const int64_t b = pair0(l, k);
