@@ -442,13 +442,10 @@ static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, co
         StageTimer tm(ctx, "expand");
         if (launch_expand_packed(d_ao, pa, vsq, ctx->stream) != hipSuccess) return CIPSI_ERR_HIP;
     }
-    // recompute the first half (tpqkld), then the second half for the energy
-    if ((rc = run_half(ctx, 1, n, m, vsq, cr, h, CIPSI_PREC_FP64, 0))) return rc;
-    {
-        StageTimer tm(ctx, "transpose");
-        if (launch_transpose(h, pa, pm, vsq, ctx->stream) != hipSuccess) return CIPSI_ERR_HIP;
-    }
-    if ((rc = run_half(ctx, 2, n, m, vsq, cr, g, CIPSI_PREC_FP64, 0))) return rc;
+    // recompute the first half (tpqkld), stored as H^T (a row of P_A per MO pair) for the
+    // second half and the gradient, then the second half for the energy
+    if ((rc = run_half(ctx, 1, n, m, vsq, cr, h, CIPSI_PREC_FP64, 0, 1))) return rc;
+    if ((rc = run_half(ctx, 2, n, m, h, cr, g, CIPSI_PREC_FP64, 0))) return rc;
     // h_MO = C^T h C (tmono) as a one-matrix sandwich
     {
         HalfArgs a{};
@@ -482,8 +479,7 @@ static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, co
     if (!d_grad) return CIPSI_OK;
 
     // gradient: Gamma expanded to the full P_M x P_M (row b = Gamma_b packed), then
-    // 2 sum_b m(b) H_b C Gamma_b + 2 h C gs from the rows of H^T still in vsq, and the
-    // reduction
+    // 2 sum_b m(b) H_b C Gamma_b + 2 h C gs from the rows of H^T, and the reduction
     if ((rc = ctx_alloc(ctx, ctx->adj_gh, sizeof(double) * pm * pm))) return rc;
     double* gfull = (double*)ctx->adj_gh.ptr;
     {
@@ -493,7 +489,7 @@ static int energy_impl(cipsi_ijkl_ctx* ctx, int n, int m, const double* d_ao, co
     }
     {
         StageTimer tm(ctx, "grad_mfma");
-        if (launch_grad(gp, vsq, pa, gfull, pm, cr, d_hp, gs, n, m, gpart, ctx->stream) !=
+        if (launch_grad(gp, h, pa, gfull, pm, cr, d_hp, gs, n, m, gpart, ctx->stream) !=
             hipSuccess)
             return CIPSI_ERR_HIP;
     }
