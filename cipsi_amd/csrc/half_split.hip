@@ -158,11 +158,13 @@ __global__ void k_split_cslices(const double* __restrict__ cr, int ldm, int np, 
 }
 
 // Row exponents of the symmetric matrices Z_b (rows of the packed source): one
-// work-group per matrix.  Each wave walks whole packed rows r (elements (r, s), s <= r,
-// contiguous), takes the row's maximum |x| by a wave reduction, and hands each element
-// with s < r to row s by an LDS atomic max on distinct addresses.  Maxima are kept as
-// the bits of |x| (ordered like the values), so the exponent is that of the largest
-// element, as element-wise exponents would give.
+// work-group per matrix.  Each wave takes two packed rows r at a time (elements (r, s),
+// s <= r, contiguous) and loads all of both rows before it reduces them, so that eight
+// loads per lane are in flight; the row's maximum |x| comes from a wave reduction, and each
+// element with s < r is handed to row s by an LDS atomic max on distinct addresses.
+// Addresses past a row are clamped to its last element and their values dropped, so every
+// load is unconditional.  Maxima are kept as the bits of |x| (ordered like the values), so
+// the exponent is that of the largest element, as element-wise exponents would give.
 __global__ __launch_bounds__(256) void k_split_rowexp(const double* __restrict__ src,
                                                       int64_t src_ld, int n, int kp,
                                                       int* __restrict__ rexp) {
@@ -172,19 +174,38 @@ __global__ __launch_bounds__(256) void k_split_rowexp(const double* __restrict__
     mx[threadIdx.x] = 0ull;
     __syncthreads();
     const unsigned long long* row = (const unsigned long long*)(src + b * src_ld);
-    for (int r = w; r < n; r += 4) {
-        const unsigned long long* seg = row + pair0(r, 0);
-        unsigned long long m = 0ull;
-        for (int s = lane; s <= r; s += 64) {
-            const unsigned long long v = seg[s] & 0x7FFFFFFFFFFFFFFFull;
-            m = v > m ? v : m;
-            if (s < r) atomicMax(&mx[s], v);
+    constexpr unsigned long long kAbs = 0x7FFFFFFFFFFFFFFFull;
+    for (int r0 = w; r0 < n; r0 += 8) {
+        unsigned long long v[2][4];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int r = r0 + 4 * q;
+            const int rc = r < n ? r : n - 1;
+            const unsigned long long* seg = row + pair0(rc, 0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int s = lane + 64 * k;
+                v[q][k] = seg[s <= rc ? s : rc];
+            }
         }
-        for (int o = 32; o >= 1; o >>= 1) {
-            const unsigned long long t = __shfl_xor(m, o);
-            m = t > m ? t : m;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int r = r0 + 4 * q;
+            if (r >= n) continue;                          // the same for the whole wave
+            unsigned long long m = 0ull;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int s = lane + 64 * k;
+                const unsigned long long a = s <= r ? (v[q][k] & kAbs) : 0ull;
+                m = a > m ? a : m;
+                if (s < r) atomicMax(&mx[s], a);
+            }
+            for (int o = 32; o >= 1; o >>= 1) {
+                const unsigned long long t = __shfl_xor(m, o);
+                m = t > m ? t : m;
+            }
+            if (lane == 0) atomicMax(&mx[r], m);
         }
-        if (lane == 0) atomicMax(&mx[r], m);
     }
     __syncthreads();
     for (int r = threadIdx.x; r < kp; r += 256) {
