@@ -8,6 +8,8 @@
 //   half 2       per MO pair: C^T Y C, ij <= kl        (tijkld.f)      -> packed MO stream
 #include <hip/hip_runtime.h>
 
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -578,6 +580,13 @@ class PageToucher {
         if (overlaps(p, bytes, in1, n1) || (in2 && overlaps(p, bytes, in2, n2)) ||
             (in3 && overlaps(p, bytes, in3, n3)))
             return;
+        // Ask for transparent huge pages over the whole 2 MiB pages inside the buffer: a
+        // fresh 3.35 GB output then faults about 1,600 times instead of 820,000 (advice
+        // only; without huge pages the touch below faults the small pages as before).
+        constexpr uintptr_t kHuge = uintptr_t(2) << 20;
+        const uintptr_t h0 = ((uintptr_t)p + kHuge - 1) & ~(kHuge - 1);
+        const uintptr_t h1 = ((uintptr_t)p + bytes) & ~(kHuge - 1);
+        if (h1 > h0) (void)madvise((void*)h0, h1 - h0, MADV_HUGEPAGE);
         constexpr size_t kPage = 4096;
         const size_t part = ((bytes / kThreads) + kPage - 1) & ~(kPage - 1);
         for (size_t off = 0; off < bytes && used_ < kThreads; off += part) {
