@@ -4,7 +4,8 @@
 // one process per geometry (Autocip_13/autocip13.f90:525-567, ijkl.f:318).  Here the
 // whole scan is one hipGraph launch:
 //   pad_c (all)  ->  expand (all)  ->  half 1, one launch per row class, storing H^T
-//   ->  half 2, one launch per row class
+//   ->  half 2, one launch per row class (with the 8-wave, depth-16 variant: the paired
+//       kernel of the single transform, its items numbered over the class)
 // Every launch is grouped over the geometries: its kernels read the per-geometry
 // arguments from device tables, so the captured graph does not depend on the caller's
 // pointers and is reused by every run of the plan (a run uploads the tables, then
@@ -40,7 +41,13 @@ struct ClassPlan {
     int first = 0;               // offset of this class in the per-class-ordered tables
     int v1 = 3, v2 = 3;          // tile variants of the two halves
     int64_t items1 = 0, items2 = 0;
+    int64_t pair_first = 0;      // this class's paired half-2 items in pair_items
+    int64_t npair = 0;
 };
+
+// Half 2 of a class runs paired (k_half2_pair_group) when its variant is the one the
+// single transform runs paired.
+bool paired_half2(const ClassPlan& cp) { return cp.v2 == kPairedHalf2Variant; }
 
 }  // namespace
 
@@ -58,6 +65,7 @@ struct cipsi_ijkl_batch {
     void* tile_tables = nullptr; // half-2 tile prefix tables, 4 column widths x sum(m + 1)
     std::vector<int64_t> tt_off; // offset of geometry g's table (per width: + w * tt_stride)
     int64_t tt_stride = 0;
+    void* pair_items = nullptr;  // paired half-2 items of every class (GroupPairItem)
     size_t tables_bytes = 0;
     size_t off_layout = 0, off_h1 = 0, off_h2 = 0, off_s1 = 0, off_s2 = 0;
     void* host_tables = nullptr; // pinned staging of the tables
@@ -175,7 +183,7 @@ void fill_tables(cipsi_ijkl_batch* b, const double* const* ao, const double* con
             st2[i + 1] = st2[i] + tot;
         }
         cp.items1 = st1[cp.geoms.size()];
-        cp.items2 = st2[cp.geoms.size()];
+        cp.items2 = paired_half2(cp) ? cp.npair : st2[cp.geoms.size()];
     }
 }
 
@@ -205,6 +213,11 @@ GroupLaunch group_args(const cipsi_ijkl_batch* b, size_t k, int half) {
     gl.ngeom = (int)cp.geoms.size();
     gl.row_class = cp.row_class;
     gl.variant = half == 1 ? cp.v1 : cp.v2;
+    if (half == 2 && paired_half2(cp)) {
+        gl.paired = 1;
+        gl.pair_items = (const GroupPairItem*)b->pair_items + cp.pair_first;
+        gl.npair = cp.npair;
+    }
     const int64_t items = half == 1 ? cp.items1 : cp.items2;
     gl.grid = xcd_grid(items);
     return gl;
@@ -218,6 +231,7 @@ int prepare_class(cipsi_ijkl_ctx* ctx, ClassPlan& cp) {
     hipError_t e = launch_half_fp64_group(gl, nullptr);
     if (e != hipSuccess) return fail(ctx, e, "batch prepare");
     gl.variant = cp.v2;
+    gl.paired = paired_half2(cp);
     e = launch_half_fp64_group(gl, nullptr);
     if (e != hipSuccess) return fail(ctx, e, "batch prepare");
     return CIPSI_OK;
@@ -315,6 +329,7 @@ int cipsi_ijkl_batch_destroy(cipsi_ijkl_ctx* ctx, cipsi_ijkl_batch* b) {
     if (b->work) (void)hipFree(b->work);
     if (b->tables) (void)hipFree(b->tables);
     if (b->tile_tables) (void)hipFree(b->tile_tables);
+    if (b->pair_items) (void)hipFree(b->pair_items);
     if (b->host_tables) (void)hipHostFree(b->host_tables);
     if (b->copied) (void)hipEventDestroy(b->copied);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
@@ -388,10 +403,25 @@ int cipsi_ijkl_batch_create(cipsi_ijkl_ctx* ctx, int n_geom, const int* n_ao, co
             for (int l = 0; l < b->m[g]; ++l)
                 p[l + 1] = p[l] + (int64_t)(l + 1) * ((l + kLts[w]) / kLts[w]);
         }
+    // paired half-2 items, class after class (they depend on the sizes only)
+    std::vector<GroupPairItem> pit;
+    {
+        std::vector<PairItem> one;
+        for (auto& cp : b->classes) {
+            cp.pair_first = (int64_t)pit.size();
+            for (size_t i = 0; i < cp.geoms.size(); ++i) {
+                one.clear();
+                build_pair_items(0, b->m[cp.geoms[i]], one);
+                for (const PairItem& it : one) pit.push_back(GroupPairItem{it, (int64_t)i});
+            }
+            cp.npair = (int64_t)pit.size() - cp.pair_first;
+        }
+    }
     hipError_t e;
     if ((e = hipMalloc(&b->work, b->work_bytes)) != hipSuccess ||
         (e = hipMalloc(&b->tables, b->tables_bytes)) != hipSuccess ||
         (e = hipMalloc(&b->tile_tables, sizeof(int64_t) * tt.size())) != hipSuccess ||
+        (e = hipMalloc(&b->pair_items, sizeof(GroupPairItem) * pit.size())) != hipSuccess ||
         (e = hipHostMalloc(&b->host_tables, b->tables_bytes, hipHostMallocDefault)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&b->copied, hipEventDisableTiming)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking)) != hipSuccess) {
@@ -401,6 +431,8 @@ int cipsi_ijkl_batch_create(cipsi_ijkl_ctx* ctx, int n_geom, const int* n_ao, co
     }
     std::memset(b->host_tables, 0, b->tables_bytes);
     if ((e = hipMemcpy(b->tile_tables, tt.data(), sizeof(int64_t) * tt.size(),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->pair_items, pit.data(), sizeof(GroupPairItem) * pit.size(),
                        hipMemcpyHostToDevice)) != hipSuccess) {
         rc = fail(ctx, e, "batch create");
         cipsi_ijkl_batch_destroy(ctx, b);

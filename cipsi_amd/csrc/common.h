@@ -129,9 +129,17 @@ constexpr int kPairedHalf2Variant = 2;
 // Row class (compile-time row-block count) used for a padded AO dimension np.
 int row_class(int np);
 
+// A paired half-2 item of a grouped launch: its pieces belong to transform geom.
+struct GroupPairItem {
+    PairItem p;
+    int64_t geom;          // index into GroupLaunch::geoms
+};
+
 // A grouped launch of one half over several transforms whose padded AO dimension
 // falls in one row class.  geoms[g] are the per-transform arguments (device
 // array); item_start (device, ngeom + 1 entries) numbers their work items.
+// paired != 0 (half 2, variant kPairedHalf2Variant): the work items are pair_items
+// (device, npair entries), built by build_pair_items per transform.
 struct GroupLaunch {
     const HalfArgs* geoms;
     const int64_t* item_start;
@@ -139,6 +147,9 @@ struct GroupLaunch {
     int row_class;
     int variant;
     int64_t grid;    // > 0: launch with this grid (>= item total); <= 0: prepare only
+    int paired;
+    const GroupPairItem* pair_items;
+    int64_t npair;
 };
 hipError_t launch_half_fp64_group(const GroupLaunch& gl, hipStream_t st);
 // XCD-aware work-item mapping of the half-transform kernels.  The hardware deals

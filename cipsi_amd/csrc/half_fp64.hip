@@ -485,6 +485,34 @@ static hipError_t launch_pair_rows(const HalfArgs& a, const PairItem* items, int
     return hipGetLastError();
 }
 
+// Paired half 2 grouped over the transforms of one row class: an item names its transform
+// (an index into geoms) besides its pieces, and the pieces of one item come from one
+// transform.
+template <int NRB>
+__global__ __launch_bounds__(512, 1) void k_half2_pair_group(const HalfArgs* __restrict__ geoms,
+                                                            const GroupPairItem* __restrict__ items,
+                                                            int64_t nitems) {
+    const int64_t item = xcd_item(blockIdx.x);
+    if (item >= nitems) return;
+    const GroupPairItem gi = items[item];
+    const HalfArgs A = geoms[gi.geom];
+    pair_tile<NRB>(A, gi.p);
+}
+
+template <int NRB>
+static hipError_t launch_group_pair_rows(const GroupLaunch& gl, hipStream_t st) {
+    const void* fn = (const void*)k_half2_pair_group<NRB>;
+    const size_t shmem = sizeof(double) * 4 * 16 * (NRB * 16 + 1);
+    if (gl.grid <= 0)
+        return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (!gl.pair_items || gl.grid > 0x7fffffff || gl.grid != xcd_grid(gl.grid) ||
+        gl.grid < gl.npair)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_half2_pair_group<NRB>), dim3((unsigned)gl.grid), dim3(512), shmem, st,
+                       gl.geoms, gl.pair_items, gl.npair);
+    return hipGetLastError();
+}
+
 
 template <int NW, int NRB, int KC, int MINB>
 __global__ __launch_bounds__(NW * 64, MINB) void k_half_fp64(HalfArgs A) {
@@ -727,6 +755,15 @@ void build_pair_items(int l0, int l1, std::vector<PairItem>& out) {
 }
 
 hipError_t launch_half_fp64_group(const GroupLaunch& gl, hipStream_t st) {
+    if (gl.paired) {
+        if (gl.variant != kPairedHalf2Variant) return hipErrorInvalidValue;
+        switch (gl.row_class) {
+#define CIPSI_CASE(R) case R: return launch_group_pair_rows<R>(gl, st);
+            CIPSI_ROW_CLASSES(CIPSI_CASE)
+#undef CIPSI_CASE
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (gl.row_class) {
 #define CIPSI_CASE(R) case R: return launch_group_rows<R>(gl, st);
         CIPSI_ROW_CLASSES(CIPSI_CASE)

@@ -75,6 +75,35 @@ def test_batch_matches_single_path_bitwise(ctx):
     plan.close()
 
 
+def test_paired_half2_in_the_batch(ctx):
+    """Half 2 with the 8-wave, depth-16 variant runs the paired kernel over a whole row class:
+    MO pairs of more than 8 column blocks split into a full piece and a remainder, and two
+    pieces of one geometry share a work-group (N_MO = 256: remainders of 8 blocks).  Two
+    geometries share row class 10.  Every plan with that half-2 variant gives the bits of the
+    single transform, eager and from the graph; so does an unpaired half 2 afterwards."""
+    from cipsi_amd import ijkl
+    sizes = [(150, 144), (160, 139), (40, 33), (250, 240), (256, 256), (70, 62)]
+    print(f"[batch-paired] sizes={sizes}", flush=True)
+    aos, cs, _ = make_geoms(sizes, 950)
+    ref = single(ctx, aos, cs)
+    plan = ijkl.BatchPlan(ctx, [n for n, _ in sizes], [m for _, m in sizes])
+    shapes = ijkl.variants()
+    paired = shapes.index((8, 16, 1))
+    unpaired = shapes.index((8, 32, 1))
+    for v2 in (paired, unpaired):
+        for k in range(len(plan.classes())):
+            plan.set_class_plan(k, paired, v2)
+        assert all(c[3] == v2 for c in plan.classes())
+        for use_graph in (False, True):
+            out = outputs(sizes)
+            plan.run(aos, cs, out, use_graph=use_graph)
+            torch.cuda.synchronize()
+            for g, (o, r) in enumerate(zip(out, ref)):
+                assert torch.equal(o, r), f"geometry {g} {sizes[g]} half2 {v2} graph={use_graph}"
+        del out
+    plan.close()
+
+
 def test_batch_autotune_and_class_plans(ctx):
     from cipsi_amd import ijkl
     sizes = MIXED[:6]
