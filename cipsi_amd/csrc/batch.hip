@@ -35,6 +35,10 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 int round16(int x) { return (x + 15) & ~15; }
 
+// Row stride of a geometry's expanded pair matrix: P_A rounded up to whole 128-byte lines,
+// so that every row of the expand's stores and of half 1's Z panel loads starts on a line.
+int64_t vsq_ld(int64_t pa) { return (pa + 15) & ~(int64_t)15; }
+
 struct ClassPlan {
     int row_class = 0;
     std::vector<int> geoms;      // geometry indices, in scan order
@@ -128,6 +132,7 @@ void fill_tables(cipsi_ijkl_batch* b, const double* const* ao, const double* con
         L.ldc = ldc[g];
         L.pa = npairs(L.n);
         L.pm = npairs(L.m);
+        L.ldv = vsq_ld(L.pa);
         L.padc_start = padc;
         L.expand_start = expd;
         L.trans_start = 0;                       // half 1 stores H^T: no transpose
@@ -159,7 +164,8 @@ void fill_tables(cipsi_ijkl_batch* b, const double* const* ao, const double* con
             a.src_ld = L.pa;
             a.b_begin = 0;
             HalfArgs x = a;                      // half 1: AO pairs of the expanded block,
-            x.dst = L.h;                         // stored as H^T
+            x.src_ld = L.ldv;                    // rows ldv apart, stored as H^T
+            x.dst = L.h;
             x.store_t = 1;
             x.half = 1;
             x.nb = L.pa;
@@ -314,7 +320,7 @@ int64_t cipsi_ijkl_batch_workspace_bytes(int n_geom, const int* n_ao, const int*
     int64_t t = 0;
     for (int g = 0; g < n_geom; ++g) {
         const int64_t pa = npairs(n_ao[g]), pm = npairs(n_mo[g]);
-        t += align256(8 * pa * pa) + align256(8 * pa * pm) +
+        t += align256(8 * pa * vsq_ld(pa)) + align256(8 * pa * pm) +
              align256(8 * (size_t)cr_pair_size(round16(n_mo[g])));
     }
     return t;
@@ -368,7 +374,7 @@ int cipsi_ijkl_batch_create(cipsi_ijkl_ctx* ctx, int n_geom, const int* n_ao, co
     for (int g = 0; g < n_geom; ++g) {
         const int64_t pa = npairs(b->n[g]), pm = npairs(b->m[g]);
         b->off_vsq.push_back(off);
-        off += align256(8 * pa * pa);
+        off += align256(8 * pa * vsq_ld(pa));
         b->off_h.push_back(off);
         off += align256(8 * pa * pm);
         b->off_cr.push_back(off);

@@ -173,10 +173,11 @@ int64_t xcd_grid(int64_t items);
 struct LayoutGeom {
     const double* ao;      // packed AO block
     const double* c;       // caller's C, column-major, ld ldc
-    double* vsq;           // full AO supermatrix, later H^T
+    double* vsq;           // full AO supermatrix, rows ldv apart
     double* h;             // H
     double* cr;            // zero-padded C in 16-column blocks (cr_index)
     int64_t pa, pm;
+    int64_t ldv;           // row stride of vsq (pa rounded up to 16 doubles)
     int n, m, np, mp, ldc;
     int64_t expand_start;  // first expand tile of this transform
     int64_t trans_start;   // first transpose tile

@@ -364,11 +364,12 @@ hipError_t launch_scan_input(const double* p, int64_t n, const int32_t* labels,
 // The packed AO block stores each unordered pair-of-pairs once; the sandwich kernel
 // wants row pq of the symmetric pair matrix contiguous.  64 x 64 tiles over the
 // lower triangle: read the packed tile (rows contiguous), write it straight and
-// transposed through LDS, both coalesced.
+// transposed through LDS, both coalesced.  Row a of the pair matrix starts at vsq + a * ld
+// (ld >= pa; the grouped expand pads rows to whole 128-byte lines).
 constexpr int XT = kExpandTile;
 
 __device__ __forceinline__ void expand_tile(const double* __restrict__ v, int64_t pa,
-                                            double* __restrict__ vsq, int64_t t) {
+                                            double* __restrict__ vsq, int64_t ld, int64_t t) {
     __shared__ double tile[XT][XT + 1];
     const int64_t I = pair_row(t);
     const int64_t J = t - pair0(I, 0);
@@ -379,19 +380,19 @@ __device__ __forceinline__ void expand_tile(const double* __restrict__ v, int64_
         double val = 0.0;
         if (a < pa && b < pa) val = v[pairsym(a, b)];
         tile[yy][tx] = val;
-        if (a < pa && b < pa) vsq[a * pa + b] = val;
+        if (a < pa && b < pa) vsq[a * ld + b] = val;
     }
     if (I == J) return;
     __syncthreads();
     for (int yy = ty; yy < XT; yy += 4) {
         int64_t b = b0 + yy, a = a0 + tx;
-        if (a < pa && b < pa) vsq[b * pa + a] = tile[tx][yy];
+        if (a < pa && b < pa) vsq[b * ld + a] = tile[tx][yy];
     }
 }
 
 __global__ __launch_bounds__(256) void k_expand_packed(const double* __restrict__ v, int64_t pa,
                                                        double* __restrict__ vsq, int64_t t0) {
-    expand_tile(v, pa, vsq, t0 + blockIdx.x);
+    expand_tile(v, pa, vsq, pa, t0 + blockIdx.x);
 }
 
 int64_t expand_tiles(int64_t pa) { return npairs((pa + XT - 1) / XT); }
@@ -528,7 +529,7 @@ __device__ __forceinline__ int find_geom(const LayoutGeom* __restrict__ g, int n
 __global__ __launch_bounds__(256) void k_group_expand(const LayoutGeom* __restrict__ g, int ngeom) {
     const int64_t t = blockIdx.x;
     const int k = find_geom(g, ngeom, t, [](const LayoutGeom& x) { return x.expand_start; });
-    expand_tile(g[k].ao, g[k].pa, g[k].vsq, t - g[k].expand_start);
+    expand_tile(g[k].ao, g[k].pa, g[k].vsq, g[k].ldv, t - g[k].expand_start);
 }
 
 __global__ __launch_bounds__(256) void k_group_transpose(const LayoutGeom* __restrict__ g, int ngeom) {

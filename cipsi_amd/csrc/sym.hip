@@ -806,7 +806,7 @@ extern "C" int cipsi_ijkl_sym_transform_labels(
         y.tri_out = B.ykl; y.half = 2; y.diag = B.yijkl; y.nij = B.nijc;
         LayoutGeom& g = tg[b];
         std::memset(&g, 0, sizeof(g));
-        g.h = d_h + B.h_off; g.vsq = d_ht + B.h_off; g.pa = B.npq; g.pm = B.nlk;
+        g.h = d_h + B.h_off; g.vsq = d_ht + B.h_off; g.pa = B.npq; g.pm = B.nlk; g.ldv = B.npq;
         g.trans_start = ttiles;
         ttiles += transpose_tiles(B.npq, B.nlk);
     }
